@@ -1,0 +1,95 @@
+"""Write the golden fixtures of tests/golden/ from the reference itself.
+
+Runs the reference STEP_7 program, compiled from its own sources by
+`make -C oracle ref` (needs the reference checkout at /root/reference, so it
+runs only in the build container), through oracle/ref_harness.f90, and
+stores its outputs as small .npz files (float64, no pickles) plus a
+MANIFEST of sha256 sums.  The fixtures are data: inputs (the input.dat
+records) and the reference's outputs (field snapshots and the text files
+its diagnostics write).
+
+    python tests/golden/make_golden.py
+
+Each fixture holds:
+  params      JSON of bgk_pcm_amd.params.Params for the run
+  nsteps      steps run
+  init        (F, Ny+2, Nx+2) state after initialization
+  final       (F, Ny+2, Nx+2) state after nsteps
+  s{step}_{k} (F, Ny+2, Nx+2) state before (k=0) / after stage k=1..8 of the
+              window steps
+  fields      names of the F stored fields (a subset of the 46)
+  text_<fn>   text files written by the reference diagnostics
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import pathlib
+import sys
+from dataclasses import asdict
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parents[1]))
+sys.path.insert(0, str(HERE.parent))
+
+from bgk_pcm_amd.params import Params  # noqa: E402
+from oracle_py import FIELD_NAMES, run_ref  # noqa: E402
+
+STATE = [f"f{k}" for k in range(9)] + [f"g{k}" for k in range(9)] + ["rho", "u", "v", "T", "phi"]
+ALL = list(FIELD_NAMES)
+
+# name: (params, nsteps, window steps, stages kept in the window, fields, diag_every)
+CASES = {
+    # no-slip cavity (UTIL/input.dat.0 physics), every stage of one step
+    "stages_cav14x10": (Params(nx=14, ny=10), 40, [40], range(9), ALL, 0),
+    # every boundary type of BC_f (BC_f.f90:9-196) at least once
+    "bc_mixA_12x10": (Params(nx=12, ny=10, re=2.0, length=10.0, west_bc=1, east_bc=3,
+                             north_bc=5, south_bc=6), 30, [30], range(3), ALL, 0),
+    "bc_mixB_12x10": (Params(nx=12, ny=10, re=2.0, length=10.0, west_bc=2, east_bc=3,
+                             north_bc=1, south_bc=4), 30, [30], range(3), ALL, 0),
+    "bc_mixC_12x10": (Params(nx=12, ny=10, re=2.0, length=10.0, west_bc=3, east_bc=0,
+                             north_bc=6, south_bc=0), 30, [30], range(3), ALL, 0),
+    "bc_periodic_12x10": (Params(nx=12, ny=10, re=2.0, length=10.0, west_bc=0, east_bc=0,
+                                 north_bc=0, south_bc=0), 30, [30], range(3), ALL, 0),
+    # long runs: final state and the diagnostics text
+    "run_cav32_1000": (Params(nx=32, ny=32), 1000, [], [], STATE, 1000),
+    "run_cav60x16_u0_400": (Params(nx=60, ny=16, re=5.0, length=16.0), 400, [], [], STATE, 100),
+    "run_clean_45x7_300": (Params(nx=45, ny=7, rounding=1), 300, [], [], STATE, 0),
+}
+
+
+def main() -> None:
+    manifest = []
+    for name, (p, nsteps, window, stages, fields, diag_every) in CASES.items():
+        variant = "step7_r8" if p.rounding == 1 else "step7"
+        first = window[0] if window else 1
+        last = window[-1] if window else 0
+        r = run_ref(p, nsteps, first, last, variant=variant, diag_every=diag_every)
+        idx = [FIELD_NAMES.index(f) for f in fields]
+        data = {
+            "params": np.array(json.dumps(asdict(p))),
+            "nsteps": np.array(nsteps),
+            "fields": np.array(fields),
+            "init": r["init"][idx],
+            "final": r["final"][idx],
+        }
+        for s in window:
+            for k in stages:
+                data[f"s{s}_{k}"] = r[("s", s, k)][idx]
+        for key, val in r.items():
+            if isinstance(key, str) and key.startswith("text:"):
+                data["text_" + key[5:].replace(".", "_").replace("-", "_")] = np.array(val)
+        if not np.all(np.isfinite(data["final"])):
+            raise RuntimeError(f"{name}: non-finite reference output")
+        path = HERE / f"{name}.npz"
+        np.savez_compressed(path, **data)
+        digest = hashlib.sha256(path.read_bytes()).hexdigest()
+        manifest.append(f"{digest}  {path.name}")
+        print(f"{name}: {path.stat().st_size / 1024:.0f} KiB")
+    (HERE / "MANIFEST.sha256").write_text("\n".join(manifest) + "\n")
+
+
+if __name__ == "__main__":
+    main()
