@@ -1,0 +1,209 @@
+// bgk_device.h -- device-side building blocks of the BGK_PCM step.
+//
+// The arithmetic here reproduces the reference STEP_7 expressions operand by
+// operand (Fortran evaluates a*b*c as (a*b)*c and a+b-c as (a+b)-c, and so
+// does C++), so that translation units compiled with -ffp-contract=off give
+// the reference's FP64 results bit for bit.  The FP32-storage kernel reuses
+// the lattice constants only.
+#ifndef BGK_DEVICE_H
+#define BGK_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bgk_tanh.h"
+
+namespace bgk {
+
+// D2Q9 velocities, STEP_7/initialization.f90:52-70: 0 rest, 1 E, 2 N, 3 W,
+// 4 S, 5 NE, 6 NW, 7 SW, 8 SE; OPP[k] is the reversed direction.
+constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+
+// Lattice geometry of a handle: plane pointers address node (0,0); node
+// (i,j) is at [j*pitch + i].
+struct Geom {
+  int nx, ny;
+  int64_t pitch;
+};
+
+struct Pops64 {
+  double* p[9];
+};
+struct Pops32 {
+  float* p[9];
+};
+
+// Scalars of module `shared` the step reads (STEP_7/setup.f90:79-105,
+// initialization.f90:5-12, forces.f90:12-13, phase_field.f90:15-19).
+struct Consts {
+  double w[9];
+  double coef[9];  // (0.5*w_k)/(cs2*cs2), collision.f90:94-130
+  double cs2;
+  double omega_f, omega_g;
+  double rho0, rho1;
+  double t0, tin, t_in_sol;
+  double u0, v0;
+  double delta_t;   // T0 - T_in_sol
+  double dt_st;     // delta_T / stefan_number
+  double beta;      // (1 - rho1/rho0)/delta_T, FP32-rounded under STEP7 rounding
+  double grav;
+  double t_width;
+};
+
+// Second-order equilibria of the nine directions for moment m (rho or T),
+// STEP_7/collision.f90:17-41.
+__device__ __forceinline__ void equilibria(const Consts& c, double m, double u, double v,
+                                           double eq[9]) {
+  const double uu = u * u;
+  const double vv = v * v;
+  const double uv = 1.5 * (uu + vv);
+  const double upv = u + v;
+  const double umv = u - v;
+  eq[0] = c.w[0] * m * (1.0 - uv);
+  eq[1] = c.w[1] * m * (1.0 + 3.0 * u + 4.5 * uu - uv);
+  eq[2] = c.w[2] * m * (1.0 + 3.0 * v + 4.5 * vv - uv);
+  eq[3] = c.w[3] * m * (1.0 - 3.0 * u + 4.5 * uu - uv);
+  eq[4] = c.w[4] * m * (1.0 - 3.0 * v + 4.5 * vv - uv);
+  eq[5] = c.w[5] * m * (1.0 + 3.0 * upv + 4.5 * upv * upv - uv);
+  eq[6] = c.w[6] * m * (1.0 - 3.0 * umv + 4.5 * umv * umv - uv);
+  eq[7] = c.w[7] * m * (1.0 - 3.0 * upv + 4.5 * upv * upv - uv);
+  eq[8] = c.w[8] * m * (1.0 + 3.0 * umv + 4.5 * umv * umv - uv);
+}
+
+// One equilibrium direction, same operand order as equilibria().
+__device__ __forceinline__ double equilibrium_k(const Consts& c, int k, double m, double u,
+                                                double v) {
+  const double uu = u * u;
+  const double vv = v * v;
+  const double uv = 1.5 * (uu + vv);
+  const double upv = u + v;
+  const double umv = u - v;
+  switch (k) {
+    case 0: return c.w[0] * m * (1.0 - uv);
+    case 1: return c.w[1] * m * (1.0 + 3.0 * u + 4.5 * uu - uv);
+    case 2: return c.w[2] * m * (1.0 + 3.0 * v + 4.5 * vv - uv);
+    case 3: return c.w[3] * m * (1.0 - 3.0 * u + 4.5 * uu - uv);
+    case 4: return c.w[4] * m * (1.0 - 3.0 * v + 4.5 * vv - uv);
+    case 5: return c.w[5] * m * (1.0 + 3.0 * upv + 4.5 * upv * upv - uv);
+    case 6: return c.w[6] * m * (1.0 - 3.0 * umv + 4.5 * umv * umv - uv);
+    case 7: return c.w[7] * m * (1.0 - 3.0 * upv + 4.5 * upv * upv - uv);
+    default: return c.w[8] * m * (1.0 + 3.0 * umv + 4.5 * umv * umv - uv);
+  }
+}
+
+template <bool STEP7>
+__device__ __forceinline__ double round_tmp(double x) {
+  // STEP7: the value is stored into an implicitly typed default-REAL
+  // variable (ft0..ft8, gt0..gt8), i.e. rounded to FP32 to nearest.
+  if constexpr (STEP7) return (double)(float)x;
+  return x;
+}
+
+// Regularised BGK collision of one population set, STEP_7/collision.f90:44-150.
+// m = rho (flow) or T (temperature); pop = fp or gp; out = f or g.
+template <bool STEP7>
+__device__ __forceinline__ void regularised_bgk(const Consts& c, double omega, double m,
+                                                double u, double v, const double pop[9],
+                                                double out[9]) {
+  double eq[9];
+  equilibria(c, m, u, v, eq);
+  const double a = -c.cs2;
+  const double b = 1.0 - c.cs2;
+  double d[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) d[k] = pop[k] - eq[k];
+  const double pxx = a * d[0] + b * d[1] + a * d[2] + b * d[3] + a * d[4] + b * d[5] +
+                     b * d[6] + b * d[7] + b * d[8];
+  const double pyy = a * d[0] + a * d[1] + b * d[2] + a * d[3] + b * d[4] + b * d[5] +
+                     b * d[6] + b * d[7] + b * d[8];
+  const double pxy = d[5] - d[6] + d[7] - d[8];
+  double t[9];
+  t[0] = round_tmp<STEP7>(eq[0] + c.coef[0] * (a * pxx + a * pyy));
+  t[1] = round_tmp<STEP7>(eq[1] + c.coef[1] * (b * pxx + a * pyy));
+  t[2] = round_tmp<STEP7>(eq[2] + c.coef[2] * (a * pxx + b * pyy));
+  t[3] = round_tmp<STEP7>(eq[3] + c.coef[3] * (b * pxx + a * pyy));
+  t[4] = round_tmp<STEP7>(eq[4] + c.coef[4] * (a * pxx + b * pyy));
+  t[5] = round_tmp<STEP7>(eq[5] + c.coef[5] * (b * pxx + b * pyy + 2.0 * pxy));
+  t[6] = round_tmp<STEP7>(eq[6] + c.coef[6] * (b * pxx + b * pyy - 2.0 * pxy));
+  t[7] = round_tmp<STEP7>(eq[7] + c.coef[7] * (b * pxx + b * pyy + 2.0 * pxy));
+  t[8] = round_tmp<STEP7>(eq[8] + c.coef[8] * (b * pxx + b * pyy - 2.0 * pxy));
+#pragma unroll
+  for (int k = 0; k < 9; ++k) out[k] = t[k] - omega * (t[k] - eq[k]);
+}
+
+// Initial macroscopic state of node (i,j), halo included, in the order
+// STEP_7/initialization.f90:15-152 assigns it (later assignments win).
+struct NodeInit {
+  double T, rho, u, v, phi;
+};
+
+__device__ __forceinline__ NodeInit initial_node(const Consts& c, const Geom& gm, int i, int j,
+                                                 int interf, int west_bc) {
+  NodeInit s;
+  if (i <= interf) {
+    s.T = c.t0;
+    s.rho = c.rho1;
+  } else {
+    s.T = c.t_in_sol;
+    s.rho = c.rho0;
+  }
+  s.u = c.u0;
+  s.v = c.v0;
+  if (i == 0) s.T = c.tin;
+  if (j == 0 || j == gm.ny + 1) {
+    s.u = 0.0;
+    s.v = 0.0;
+  }
+  if (west_bc == 3) {
+    s.u = 0.0;
+    s.v = 0.0;
+    if (i == 0) s.rho = c.rho1;
+  }
+  if (i == 0 && j >= 1 && j <= gm.ny) {
+    if (west_bc == 1) {
+      s.u = c.u0;
+      s.v = c.v0;
+    } else if (west_bc == 2) {
+      const double yj = (double)j;
+      const double np1 = (double)(gm.ny + 1);
+      s.u = -(6.0 * c.u0 / (np1 * np1)) * (yj * yj - np1 * yj);
+      s.v = c.v0;
+    }
+  }
+  s.phi = (i <= interf) ? 1.0 : -1.0;
+  return s;
+}
+
+// Phase-field update of one node, STEP_7/phase_field.f90:24-52.  tap[] are
+// phi_prec at C, E, N, W, S, NE, NW, SW, SE; returns the new phi.
+__device__ __forceinline__ double phase_update(const Consts& c, double T, const double tap[9]) {
+  // T_act = 0 and T_crit = 273 (phase_field.f90:15-16): both reaction rates
+  // take tanh of the same argument.
+  const double th = tanh_fdlibm(((T - 273.0) - 0.0) / c.t_width);
+  const double react_plus = 0.5 * (1.0 + th);
+  const double react_minus = 0.5 * (1.0 - th);
+  double a = 0.5 * (1.0 + tap[0]);
+  double kp = a * a;
+  double b = 0.5 * (1.0 - tap[0]);
+  double km = b * b;
+#pragma unroll
+  for (int m = 1; m < 9; ++m) {
+    a = 0.5 * (1.0 + tap[m]);
+    kp = kp + a * a;
+  }
+#pragma unroll
+  for (int m = 1; m < 9; ++m) {
+    b = 0.5 * (1.0 - tap[m]);
+    km = km + b * b;
+  }
+  kp = (kp / 8.0) * react_plus;
+  km = (km / 8.0) * react_minus;
+  const double pc = tap[0];
+  return pc + (0.5 * kp * (1.0 - pc) - 0.5 * km * (1.0 + pc));
+}
+
+}  // namespace bgk
+
+#endif  // BGK_DEVICE_H

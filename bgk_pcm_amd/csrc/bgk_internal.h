@@ -1,0 +1,96 @@
+// bgk_internal.h -- handle layout and kernel launchers of libbgk_pcm.
+#ifndef BGK_INTERNAL_H
+#define BGK_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bgk_pcm.h"
+#include "bgk_device.h"
+
+// One lattice resident in HBM.
+//
+// Layout: every field is a plane of (Ny+2) rows of `pitch` elements; node
+// (i,j) of a plane is at plane[j*pitch + i].  The pitch is a multiple of
+// 128 bytes and each plane starts one element before a 128-byte boundary,
+// so the first interior node of every row (i=1) is 128-byte aligned.  All
+// planes come from one allocation (`pool`).
+//
+// Populations are double-buffered: f[cur] is the reference's f (post
+// collision) and f[1-cur] its fp (post streaming); the fused step reads
+// f[cur], writes f[1-cur] and flips cur.  phi[pc] is phi and phi[1-pc]
+// phi_prec.  FP32 handles hold f32/g32 = f - w*rho0, g - w*T0 instead.
+struct bgk_handle {
+  bgk_params prm;
+  int nx, ny;
+  int64_t pitch;        // elements per row
+  int64_t plane_elems;  // elements between consecutive planes (>= (ny+2)*pitch)
+  int device;
+  hipStream_t own_stream;
+  hipStream_t stream;
+
+  void* pool;
+  size_t pool_bytes;
+
+  // FP64 state (FP64 handles)
+  double* f[2][9];
+  double* g[2][9];
+  // FP32 deviation state (FP32 handles)
+  float* f32[2][9];
+  float* g32[2][9];
+  int cur;
+
+  double* phi[2];
+  int pc;
+  double *rho, *u, *v, *T, *rho2, *u2, *v2, *T2;
+  double* teast[2];  // averaged T(Nx, 0..Ny+1) of the previous step, ping-pong
+  double* tmp;       // one FP64 plane: staging for FP32 conversions
+  double* partial;   // reduction partials
+  double* small;     // small device outputs (reductions, gathers)
+  int64_t small_elems;
+
+  bgk::Consts c;
+  bgk::Geom geom;
+  bool fused_ok;  // all four walls no-slip (BC type 4)
+};
+
+namespace bgk {
+
+// ---- per-subroutine kernels (bgk_stage.hip), FP64 handles
+hipError_t launch_initialization(bgk_handle* h);
+hipError_t launch_bc_f(bgk_handle* h);
+hipError_t launch_bc_g(bgk_handle* h);
+hipError_t launch_streaming(bgk_handle* h);
+hipError_t launch_moments_pre(bgk_handle* h);
+hipError_t launch_phase_field(bgk_handle* h);
+hipError_t launch_forces(bgk_handle* h);
+hipError_t launch_collision(bgk_handle* h);
+hipError_t launch_moments(bgk_handle* h);
+
+// ---- fused step (bgk_fused.hip / bgk_fused32.hip)
+// Copy averaged T(Nx, :) into teast[cur] before the first fused step of a call.
+hipError_t launch_teast_prologue(bgk_handle* h);
+// One fused FP64 step from buffer cur to 1-cur; write_all: write the
+// moments of every node (else only the boundary-adjacent ones).
+hipError_t launch_fused_step64(bgk_handle* h, int step_parity, bool write_all);
+hipError_t launch_fused_step32(bgk_handle* h, int step_parity, bool write_all);
+
+// ---- FP32 conversions (bgk_fused32.hip)
+hipError_t launch_dev_to_f64(bgk_handle* h, const float* src, double* dst, double base);
+hipError_t launch_f64_to_dev(bgk_handle* h, const double* src, float* dst, double base);
+// initialization of an FP32 handle (FP64 equilibria rounded to deviations)
+hipError_t launch_initialization32(bgk_handle* h);
+
+// ---- reductions and gathers (bgk_reduce.hip); results land in h->small
+// sums over the interior: out[0..2] = sum rho, sum T, sum phi
+hipError_t launch_field_sums(bgk_handle* h, double* out_dev);
+// diag sums (bgk_diag_sums layout)
+hipError_t launch_diag_sums(bgk_handle* h, double* out_dev);
+hipError_t launch_gather(bgk_handle* h, const double* plane, int n, const int32_t* ij_dev,
+                         double* out_dev);
+
+int reduce_partial_count(const bgk_handle* h);
+
+}  // namespace bgk
+
+#endif  // BGK_INTERNAL_H

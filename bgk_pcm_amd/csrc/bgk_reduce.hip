@@ -1,0 +1,160 @@
+// bgk_reduce.hip -- interior reductions and gathers for the diagnostics.
+//
+// field_check, mass_check and diag (STEP_7/field_check.f90:11-21,
+// mass_check.f90:12-29, diag.f90:63-104) sum over the interior serially.
+// Here every block owns a fixed set of rows, every thread a fixed set of
+// columns, and each accumulates a compensated (Neumaier) FP64 sum; the
+// (sum, error) pairs are then combined by error-free TwoSum in a fixed tree
+// and a second one-block pass.  The result is deterministic and within one
+// rounding of the exact sum, so it differs from the serial sum only by that
+// sum's own rounding error.  Counts are exact integers.
+#include "bgk_internal.h"
+
+namespace bgk {
+namespace {
+
+constexpr int RB = 256;        // threads per block
+constexpr int MAX_PARTS = 1024;  // first-pass blocks
+
+struct CSum {
+  double s, c;
+};
+
+__device__ __forceinline__ void cadd(CSum& a, double x) {
+  // Neumaier compensated addition
+  const double t = a.s + x;
+  if (fabs(a.s) >= fabs(x))
+    a.c += (a.s - t) + x;
+  else
+    a.c += (x - t) + a.s;
+  a.s = t;
+}
+
+__device__ __forceinline__ CSum cmerge(CSum a, CSum b) {
+  // TwoSum of the leading parts, errors added
+  const double s = a.s + b.s;
+  const double bb = s - a.s;
+  const double err = (a.s - (s - bb)) + (b.s - bb);
+  return CSum{s, a.c + b.c + err};
+}
+
+template <int N>
+__device__ __forceinline__ void block_reduce(CSum (&v)[N], CSum (*lds)[RB]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < N; ++k) lds[k][t] = v[k];
+  __syncthreads();
+  for (int s = RB / 2; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) lds[k][t] = cmerge(lds[k][t], lds[k][t + s]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = lds[k][0];
+}
+
+// Sums of three planes over the interior (rho, T, phi).
+__global__ __launch_bounds__(RB) void k_sum3(const double* __restrict__ a,
+                                             const double* __restrict__ b,
+                                             const double* __restrict__ c3, Geom gm,
+                                             CSum* __restrict__ part) {
+  __shared__ CSum lds[3][RB];
+  CSum acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  for (int j = blockIdx.x + 1; j <= gm.ny; j += gridDim.x) {
+    const int64_t r = (int64_t)j * gm.pitch;
+    for (int i = threadIdx.x + 1; i <= gm.nx; i += RB) {
+      cadd(acc[0], a[r + i]);
+      cadd(acc[1], b[r + i]);
+      cadd(acc[2], c3[r + i]);
+    }
+  }
+  block_reduce<3>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[blockIdx.x * 4 + k] = acc[k];
+}
+
+// Terms of STEP_7/diag.f90:63-104 per row/node:
+//  k=0: Ny*((Tin - T(2,j))/2)/Delta_T                  (column 2)
+//  k=1: Ny*|(-3T(1,j)+4T(2,j)-T(3,j))/2|/Delta_T        (columns 1..3)
+//  k=2: count of phi > 0;  k=3: sum of i where phi == 0
+__global__ __launch_bounds__(RB) void k_diag(const double* __restrict__ T,
+                                             const double* __restrict__ phi, Geom gm,
+                                             double tin, double dT, CSum* __restrict__ part) {
+  __shared__ CSum lds[4][RB];
+  CSum acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  long long cnt = 0, isum = 0;
+  const double dny = (double)gm.ny;
+  for (int j = blockIdx.x + 1; j <= gm.ny; j += gridDim.x) {
+    const int64_t r = (int64_t)j * gm.pitch;
+    if (threadIdx.x == 0) {
+      const double t1 = T[r + 1], t2 = T[r + 2], t3 = T[r + 3];
+      cadd(acc[0], dny * ((tin - t2) / 2.0) / dT);
+      const double gt = fabs((-(3.0 * t1) + 4.0 * t2 - t3) / 2.0);
+      cadd(acc[1], dny * gt / dT);
+    }
+    for (int i = threadIdx.x + 1; i <= gm.nx; i += RB) {
+      const double p = phi[r + i];
+      if (p > 0.0) cnt += 1;
+      if (p == 0.0) isum += i;
+    }
+  }
+  acc[2].s = (double)cnt;
+  acc[3].s = (double)isum;
+  block_reduce<4>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) part[blockIdx.x * 4 + k] = acc[k];
+}
+
+// Second pass: combine nparts partials of nk sums (stride 4).
+__global__ __launch_bounds__(RB) void k_final(const CSum* __restrict__ part, int nparts, int nk,
+                                              double* __restrict__ out) {
+  __shared__ CSum lds[4][RB];
+  CSum acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  for (int p = threadIdx.x; p < nparts; p += RB)
+    for (int k = 0; k < 4; ++k) acc[k] = cmerge(acc[k], part[p * 4 + k]);
+  block_reduce<4>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nk; ++k) out[k] = acc[k].s + acc[k].c;
+}
+
+__global__ void k_gather(const double* __restrict__ plane, int64_t pitch, int n,
+                         const int32_t* __restrict__ ij, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = plane[(int64_t)ij[2 * t + 1] * pitch + ij[2 * t]];
+}
+
+inline int nparts(const Geom& g) { return g.ny < MAX_PARTS ? g.ny : MAX_PARTS; }
+
+}  // namespace
+
+int reduce_partial_count(const bgk_handle*) { return MAX_PARTS * 4 * 2; }
+
+hipError_t launch_field_sums(bgk_handle* h, double* out_dev) {
+  const int np = nparts(h->geom);
+  CSum* part = reinterpret_cast<CSum*>(h->partial);
+  hipLaunchKernelGGL(k_sum3, dim3(np), dim3(RB), 0, h->stream, h->rho, h->T, h->phi[h->pc],
+                     h->geom, part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, h->stream, part, np, 3, out_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_sums(bgk_handle* h, double* out_dev) {
+  const int np = nparts(h->geom);
+  CSum* part = reinterpret_cast<CSum*>(h->partial);
+  hipLaunchKernelGGL(k_diag, dim3(np), dim3(RB), 0, h->stream, h->T, h->phi[h->pc], h->geom,
+                     h->c.tin, h->c.delta_t, part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, h->stream, part, np, 4, out_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(bgk_handle* h, const double* plane, int n, const int32_t* ij_dev,
+                         double* out_dev) {
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                     plane, h->geom.pitch, n, ij_dev, out_dev);
+  return hipGetLastError();
+}
+
+}  // namespace bgk

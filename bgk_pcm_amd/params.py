@@ -85,7 +85,7 @@ class BgkParamsC(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("rounding", ctypes.c_int32),
         ("device", ctypes.c_int32),
-        ("reserved1", ctypes.c_int32),
+        ("fp32_collision", ctypes.c_int32),
     ]
 
 
@@ -122,13 +122,13 @@ class Params:
     precision: int = FP64
     rounding: int = ROUND_STEP7
     device: int = 0
+    fp32_collision: int = 0
 
     def to_c(self) -> BgkParamsC:
         c = BgkParamsC()
         for f in fields(self):
             setattr(c, f.name, getattr(self, f.name))
         c.reserved0 = 0
-        c.reserved1 = 0
         return c
 
     def replace(self, **kw) -> "Params":

@@ -110,7 +110,8 @@ typedef struct bgk_params {
   int32_t precision;     /* BGK_FP64 or BGK_FP32_DEV                        */
   int32_t rounding;      /* BGK_ROUND_STEP7 or BGK_ROUND_CLEAN              */
   int32_t device;        /* HIP device ordinal                              */
-  int32_t reserved1;     /* set to 0                                        */
+  int32_t fp32_collision;/* FP32 handles: 1 = collision arithmetic in FP32,
+                            0 (default) = FP64 arithmetic on FP32 storage   */
 } bgk_params;
 
 /* Memory report (replaces STEP_7/get_mem.c:23-38, which reports ru_maxrss). */

@@ -1,0 +1,75 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point the
+header declares, its parameter struct matches the header byte for byte, and
+it refuses to run (rather than falling back to the CPU) when no GPU is
+visible."""
+import ctypes
+import os
+import pathlib
+import re
+import subprocess
+
+import pytest
+
+from bgk_pcm_amd import lattice
+from bgk_pcm_amd.params import BgkParamsC, Params
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "bgk_pcm.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(bgk_\w+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = lattice.load_library()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(lattice.SIGNATURES), set(names) ^ set(lattice.SIGNATURES)
+    out = subprocess.run(["nm", "-D", "--defined-only", str(lattice.LIB_PATH)],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(bgk_\w+)$", out, re.M))
+    assert set(names) <= exported
+
+
+def test_abi_version():
+    assert lattice.load_library().bgk_abi_version() == 1
+
+
+def test_params_struct_layout_matches_header(tmp_path):
+    fields = [f for f, _ in BgkParamsC._fields_]
+    src = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"',
+           "int main(void){", 'printf("%zu\\n", sizeof(bgk_params));']
+    src += [f'printf("%zu\\n", offsetof(bgk_params, {f}));' for f in fields]
+    src += ["return 0;}"]
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(c)], check=True)
+    vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+    assert vals[0] == ctypes.sizeof(BgkParamsC)
+    for f, off in zip(fields, vals[1:]):
+        assert getattr(BgkParamsC, f).offset == off, f
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")
+def test_no_gpu_means_error_not_fallback():
+    with pytest.raises(lattice.BgkError) as e:
+        lattice.Lattice(Params(nx=8, ny=8))
+    assert e.value.code == -5
+    assert "device" in str(e.value)
+
+
+def test_invalid_parameters_are_rejected_before_touching_the_device():
+    lib = lattice.load_library()
+    h = ctypes.c_void_p()
+    for bad in (Params(nx=0, ny=8), Params(nx=8, ny=8, west_bc=7), Params(nx=8, ny=8, east_bc=1),
+                Params(nx=8, ny=8, tau_f=0.0), Params(nx=8, ny=8, precision=3)):
+        c = bad.to_c()
+        assert lib.bgk_create(ctypes.byref(c), ctypes.byref(h)) == -1
+        assert lib.bgk_last_error()
+    assert lib.bgk_step(None, 1) == -1

@@ -1,0 +1,140 @@
+"""The HIP library against the reference's own outputs and the oracle.
+
+Bar: bit-identical FP64 results (the reference STEP_7 path is deterministic
+IEEE double arithmetic plus FP32 roundings, all reproduced on gfx950).
+
+ - every per-subroutine kernel in isolation, from reference snapshots;
+ - the initial state, whole runs through the per-subroutine kernels and
+   through the fused kernel, against the reference fixtures;
+ - the fused kernel against the oracle over more sizes, both rounding
+   policies, any split of the steps into calls, and mixed with the
+   per-subroutine path.
+"""
+import numpy as np
+import pytest
+
+from bgk_pcm_amd.lattice import FIELD_NAMES, Lattice
+from bgk_pcm_amd.params import Params
+from golden_io import INTERIOR_ONLY, Golden, compare_state, fixtures
+from oracle_py import Oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = fixtures()
+STATE = ["rho", "u", "v", "T", "phi"]
+POPS = [f"f{k}" for k in range(9)] + [f"g{k}" for k in range(9)]
+
+
+def gpu_state(lat, fields):
+    return {f: lat.download(f) for f in fields}
+
+
+def compare_interior(got, want, fields):
+    bad = []
+    for f in fields:
+        a, b = got[f][1:-1, 1:-1], want[f][1:-1, 1:-1]
+        if not np.array_equal(np.ascontiguousarray(a).view(np.uint64),
+                              np.ascontiguousarray(b).view(np.uint64)):
+            d = np.abs(a - b)
+            bad.append(f"{f}: {np.count_nonzero(a != b)} nodes differ, max|d|={d.max():.3e}")
+    return bad
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_initialization_matches_reference(name):
+    g = Golden(name)
+    print(f"case {name}: initialization", flush=True)
+    with Lattice(g.params) as lat:
+        fields = [f for f in g.fields if not f.startswith(("fp", "gp")) and not f.endswith("2")]
+        assert compare_state(gpu_state(lat, fields), g.snap("init"), fields) == []
+
+
+@pytest.mark.parametrize("name", [c for c in CASES if Golden(c).window()])
+def test_each_subroutine_kernel_in_isolation(name):
+    g = Golden(name)
+    print(f"case {name}: stage kernels", flush=True)
+    with Lattice(g.params, initialize=False) as lat:
+        for step, stages in g.window():
+            for k in stages:
+                if k == 0 or (k - 1) not in stages:
+                    continue
+                for f, a in g.snap(f"s{step}_{k - 1}").items():
+                    lat.upload(f, a)
+                lat.stage(Lattice.STAGES[k - 1])
+                bad = compare_state(gpu_state(lat, g.fields), g.snap(f"s{step}_{k}"), g.fields)
+                assert bad == [], f"step {step} {Lattice.STAGES[k - 1]}: {bad[:5]}"
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_subroutine_path_whole_run(name):
+    g = Golden(name)
+    print(f"case {name}: per-subroutine run of {g.nsteps} steps", flush=True)
+    with Lattice(g.params) as lat:
+        for _ in range(g.nsteps):
+            for s in Lattice.STAGES:
+                lat.stage(s)
+        bad = compare_state(gpu_state(lat, g.fields), g.snap("final"), g.fields)
+        assert bad == [], bad[:5]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_bgk_step_whole_run(name):
+    """bgk_step: fused kernel for the no-slip cavity, the per-subroutine
+    kernels otherwise.  Macroscopic fields are compared halo included;
+    populations on the interior (their ghosts are scratch of the fused
+    kernel)."""
+    g = Golden(name)
+    print(f"case {name}: bgk_step({g.nsteps})", flush=True)
+    with Lattice(g.params) as lat:
+        lat.step(g.nsteps)
+        want = g.snap("final")
+        got = gpu_state(lat, STATE + POPS)
+        assert compare_state(got, want, STATE) == []
+        assert compare_interior(got, want, POPS) == []
+
+
+SIZES = [(67, 45, 150, 0), (130, 33, 100, 1), (1, 1, 40, 0), (2, 3, 40, 1), (3, 2, 40, 0),
+         (97, 70, 60, 1)]
+
+
+@pytest.mark.parametrize("nx,ny,nsteps,rounding", SIZES)
+def test_fused_vs_oracle(nx, ny, nsteps, rounding):
+    p = Params(nx=nx, ny=ny, rounding=rounding, re=3.0, length=float(ny))
+    print(f"fused vs oracle {nx}x{ny} {nsteps} steps rounding={rounding}", flush=True)
+    o = Oracle(p)
+    o.step(nsteps)
+    with Lattice(p) as lat:
+        lat.step(nsteps)
+        got = gpu_state(lat, STATE + POPS)
+    want = {f: o.field(f) for f in STATE + POPS}
+    assert compare_state(got, want, STATE) == []
+    assert compare_interior(got, want, POPS) == []
+
+
+def test_any_split_of_steps_and_mixing_paths():
+    p = Params(nx=40, ny=36, re=2.0, length=36.0)
+    o = Oracle(p)
+    o.step(40)
+    with Lattice(p) as lat:
+        for n in (1, 2, 3, 5, 7):
+            lat.step(n)          # 18 fused steps in 5 calls
+        for _ in range(4):       # 4 per-subroutine steps
+            for s in Lattice.STAGES:
+                lat.stage(s)
+        lat.step(11)             # 11 fused
+        for _ in range(7):
+            lat.step(1)          # 7 single fused steps
+        got = gpu_state(lat, STATE + POPS)
+    want = {f: o.field(f) for f in STATE + POPS}
+    assert compare_state(got, want, STATE) == []
+    assert compare_interior(got, want, POPS) == []
+
+
+def test_upload_download_roundtrip_all_fields():
+    p = Params(nx=20, ny=14)
+    rng = np.random.default_rng(7)
+    with Lattice(p, initialize=False) as lat:
+        for f in FIELD_NAMES:
+            a = rng.standard_normal((p.ny + 2, p.nx + 2))
+            lat.upload(f, a)
+            assert np.array_equal(lat.download(f), a), f
