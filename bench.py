@@ -1,0 +1,237 @@
+"""Throughput of the BGK_PCM time step on one MI355X.
+
+Workload: the melting cavity of the reference benchmark input
+UTIL/input.dat.0 (no-slip walls, tau_f=1, tau_g=0.51, grav=2.5e-4,
+Stefan 0.1) on a 4096 x 4096 lattice, advanced by the fused step with FP32
+population storage (f - w*rho0, g - w*T0) and FP64 arithmetic.  One "step"
+is one time step of the whole lattice (BC_f ... moments of the reference
+time loop).  Inputs are resident in HBM before the timed region.
+
+Printed JSON line (one):
+  value        MLUPS of the FP32-storage step = Nx*Ny*steps/t/1e6
+  roofline     algorithmic HBM bytes per launch (160 B per lattice update:
+               18 FP32 populations read + written, phi FP64 read + written)
+               over the fused kernel's mean launch time, measured with HIP
+               events on the stream the kernel runs on; vs 8 TB/s
+  fp64         the same for the FP64 fused kernel (304 B per update)
+  fp32_error   relative max-norm error of the FP32-storage run against the
+               FP64 run (STEP_7 rounding), per field, after --error-steps
+  cpu_baseline the reference's own STEP_OPENMP time step (compiled from its
+               sources into oracle/_ref) timed on this host's cores on a
+               bounded sample of the same workload; the CPU oracle (a port)
+               if that build is absent
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
+                       [--error-steps E] [--cpu-steps S] [--no-cpu-baseline]
+                       [--no-fp64] [--calibrate-copy MiB]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS (D2Q9 f+g) on 1 MI355X: FP32-store vs FP64 at <=1e-5 error; % HBM peak"
+BYTES_PER_LUP = {"fp32": 18 * 4 * 2 + 8 * 2, "fp64": 18 * 8 * 2 + 8 * 2}
+FIELDS = ("rho", "u", "v", "T", "phi")
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--error-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp32-collision", action="store_true",
+                    help="FP32 collision arithmetic instead of FP64")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="after the timed region, run a device-to-device copy of this size "
+                         "(reference kernel for HBM counter scaling in profiles)")
+    return ap.parse_args()
+
+
+def cavity(n: int):
+    from bgk_pcm_amd.params import Params
+    return Params(nx=n, ny=n, iter=20000, length=float(n))
+
+
+def timed(lat, stream, steps: int, warmup: int):
+    """(kernel ms per step from HIP events, wall ms per step)."""
+    import torch
+    lat.step(warmup)
+    lat.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    lat.step(steps)
+    ev1.record(stream)
+    lat.synchronize()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return ev0.elapsed_time(ev1) / steps, wall
+
+
+def gpu_run(p, stream, steps, warmup):
+    from bgk_pcm_amd.lattice import Lattice
+    lat = Lattice(p)
+    lat.set_stream(stream.cuda_stream)
+    kms, wms = timed(lat, stream, steps, warmup)
+    fc = lat.field_check()
+    lat.close()
+    return kms, wms, fc
+
+
+def fp32_error(p64, p32, steps: int):
+    """Relative max-norm error of the FP32-storage run (p32) against the
+    FP64 run (p64) after `steps` steps from the same initial state."""
+    import numpy as np
+    from bgk_pcm_amd.lattice import Lattice
+    from bgk_pcm_amd.params import FP32_DEV, FP64
+    assert p64.precision == FP64 and p32.precision == FP32_DEV
+    out = {}
+    with Lattice(p64) as a, Lattice(p32) as b:
+        a.step(steps)
+        b.step(steps)
+        for f in FIELDS:
+            x, y = b.download(f)[1:-1, 1:-1], a.download(f)[1:-1, 1:-1]
+            if f == "phi":  # liquid fraction
+                x, y = (1 + x) / 2, (1 + y) / 2
+            out[f] = float(np.abs(x - y).max() / np.abs(y).max())
+    return out
+
+
+def cpu_baseline(p, steps: int):
+    from bgk_pcm_amd.params import write_input_dat
+    exe = ROOT / "oracle" / "_ref" / "openmp" / "ref_bench"
+    sample = f"{p.nx}x{p.ny} cavity, {steps} steps (same input as the GPU run)"
+    if exe.exists():
+        with tempfile.TemporaryDirectory(prefix="bgkcpu_") as wd:
+            write_input_dat(p, os.path.join(wd, "input.dat"))
+            r = subprocess.run([str(exe), str(steps)], cwd=wd, capture_output=True, text=True,
+                               timeout=900)
+            if r.returncode != 0:
+                raise RuntimeError(f"ref_bench failed: {r.stderr[-500:]}")
+            line = [l for l in r.stdout.splitlines() if l.startswith("nx=")][-1]
+            kv = dict(re.findall(r"(\w+)=\s*(\S+)", line))
+            return {"value": float(kv["mlups"]), "unit": "MLUPS", "cores": int(kv["threads"]),
+                    "kind": "reference",
+                    "sample": sample + "; reference STEP_OPENMP subroutines (oracle/_ref/openmp), "
+                              f"{float(kv['seconds']):.2f} s"}
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_py import Oracle
+    o = Oracle(p)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return {"value": p.nx * p.ny * steps / dt / 1e6, "unit": "MLUPS",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "kind": "port", "sample": sample + f"; C oracle with OpenMP, {dt:.2f} s"}
+
+
+def traffic_from_profile(size: int, kind: str):
+    """HBM bytes per launch measured with rocprofv3 counters, when a
+    summary for this kernel and size is committed under profiles/."""
+    f = ROOT / "profiles" / f"fused_{kind}_{size}x{size}_hbm_traffic.json"
+    if not f.exists():
+        return None
+    try:
+        return float(json.loads(f.read_text())["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from bgk_pcm_amd.params import FP32_DEV
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.Stream()
+    n = args.size
+    p = cavity(n)
+    p32 = p.replace(precision=FP32_DEV, fp32_collision=1 if args.fp32_collision else 0)
+
+    k32, w32, _ = gpu_run(p32, stream, args.steps, args.warmup)
+    lups = n * n
+    mlups32 = lups * args.steps / (w32 * 1e-3 * args.steps) / 1e6
+    bytes32 = BYTES_PER_LUP["fp32"] * lups
+    ach32 = bytes32 / (k32 * 1e-3) / 1e9
+    res = {
+        "metric": METRIC,
+        "value": round(mlups32, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(w32, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.fp32_collision else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"melting cavity (UTIL/input.dat.0 physics), {n}x{n}, fused time step, "
+                        "FP32 population storage",
+            "nx": n, "ny": n, "storage": "f32 deviations from rest equilibrium",
+            "arithmetic": "f32 collision, f64 elsewhere" if args.fp32_collision else "f64",
+            "boundaries": "no-slip (type 4) on all walls",
+            "tau_f": p.tau_f, "tau_g": p.tau_g, "grav": p.grav, "stefan_number": p.stefan_number,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(ach32, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ach32 / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profile(n, "fp32"),
+            "kernel": "k_fused32",
+            "kernel_ms": round(k32, 5),
+            "algorithmic_bytes_per_launch": bytes32,
+        },
+    }
+    if not args.no_fp64:
+        k64, w64, _ = gpu_run(p, stream, max(args.steps // 2, 1), args.warmup)
+        ach64 = BYTES_PER_LUP["fp64"] * lups / (k64 * 1e-3) / 1e9
+        res["fp64"] = {"mlups": round(lups / (w64 * 1e-3) / 1e6, 1), "ms_per_step": round(w64, 5),
+                       "kernel_ms": round(k64, 5), "achieved_gbs": round(ach64, 1),
+                       "frac": round(ach64 / HBM_PEAK_GBS, 4),
+                       "traffic": traffic_from_profile(n, "fp64")}
+        res["fp32_over_fp64"] = round(res["value"] / res["fp64"]["mlups"], 3)
+    if args.error_steps > 0:
+        res["fp32_error"] = {"steps": args.error_steps, "vs": "FP64 run, STEP_7 rounding",
+                             "rel_max_norm": fp32_error(p, p32, args.error_steps)}
+    if args.calibrate_copy > 0:
+        nb = args.calibrate_copy * 1024 * 1024 // 8
+        a = torch.ones(nb, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        res["calibration_copy_bytes"] = 2 * nb * 8
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(p, args.cpu_steps)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

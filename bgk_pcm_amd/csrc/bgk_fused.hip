@@ -34,8 +34,6 @@ struct Macro {
   double *rho, *u, *v, *T;
 };
 
-constexpr int TX = 64, TY = 4;
-
 inline Pops64 pops(double* const* p) {
   Pops64 r;
   for (int k = 0; k < 9; ++k) r.p[k] = p[k];
@@ -49,16 +47,16 @@ __global__ void k_teast_prologue(const double* __restrict__ T, double* __restric
 }
 
 template <bool STEP7>
-__global__ __launch_bounds__(TX* TY) void k_fused64(Pops64 fa, Pops64 ga, Pops64 fb, Pops64 gb,
+__global__ __launch_bounds__(TILE_W) void k_fused64(Pops64 fa, Pops64 ga, Pops64 fb, Pops64 gb,
                                                      const double* __restrict__ pa,
                                                      double* __restrict__ pb,
                                                      const double* __restrict__ te_in,
                                                      double* __restrict__ te_out, Macro m,
-                                                     Consts c, Geom gm, int write_all) {
+                                                     Consts c, Geom gm, TileGrid tg,
+                                                     int write_all) {
   const int nx = gm.nx, ny = gm.ny;
-  const int i = blockIdx.x * TX + threadIdx.x + 1;
-  const int j = blockIdx.y * TY + threadIdx.y + 1;
-  if (i > nx || j > ny) return;
+  int i, j;
+  if (!tile_node(tg, gm, &i, &j)) return;
   const int64_t P = gm.pitch;
   const int64_t q = (int64_t)j * P + i;
   const bool w_edge = (i == 1), e_edge = (i == nx), s_edge = (j == 1), n_edge = (j == ny);
@@ -215,8 +213,9 @@ hipError_t launch_teast_prologue(bgk_handle* h) {
 
 hipError_t launch_fused_step64(bgk_handle* h, int step_parity, bool write_all) {
   const Geom& g = h->geom;
-  dim3 blk(TX, TY);
-  dim3 grd((unsigned)((g.nx + TX - 1) / TX), (unsigned)((g.ny + TY - 1) / TY));
+  const TileGrid tg = make_tile_grid(g);
+  dim3 blk(TILE_W);
+  dim3 grd((unsigned)(tg.per * NXCD));
   Macro m{h->rho, h->u, h->v, h->T};
   const double* te_in = h->teast[step_parity & 1];
   double* te_out = h->teast[(step_parity + 1) & 1];
@@ -224,11 +223,11 @@ hipError_t launch_fused_step64(bgk_handle* h, int step_parity, bool write_all) {
   if (h->prm.rounding == BGK_ROUND_STEP7)
     hipLaunchKernelGGL(k_fused64<true>, grd, blk, 0, h->stream, pops(h->f[a]), pops(h->g[a]),
                        pops(h->f[b]), pops(h->g[b]), h->phi[h->pc], h->phi[1 - h->pc], te_in,
-                       te_out, m, h->c, g, write_all ? 1 : 0);
+                       te_out, m, h->c, g, tg, write_all ? 1 : 0);
   else
     hipLaunchKernelGGL(k_fused64<false>, grd, blk, 0, h->stream, pops(h->f[a]), pops(h->g[a]),
                        pops(h->f[b]), pops(h->g[b]), h->phi[h->pc], h->phi[1 - h->pc], te_in,
-                       te_out, m, h->c, g, write_all ? 1 : 0);
+                       te_out, m, h->c, g, tg, write_all ? 1 : 0);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) {
     h->cur = b;

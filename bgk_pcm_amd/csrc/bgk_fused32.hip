@@ -20,7 +20,8 @@
 namespace bgk {
 namespace {
 
-constexpr int TX = 64, TY = 4;
+constexpr int TX = 64, TY = 4;  // block of the whole-plane conversion kernels
+
 
 struct Macro {
   double *rho, *u, *v, *T;
@@ -87,16 +88,16 @@ __device__ __forceinline__ double ghost_dev(const Consts& c, int k, double m, do
 }
 
 template <typename RC>
-__global__ __launch_bounds__(TX* TY) void k_fused32(Pops32 fa, Pops32 ga, Pops32 fb, Pops32 gb,
+__global__ __launch_bounds__(TILE_W) void k_fused32(Pops32 fa, Pops32 ga, Pops32 fb, Pops32 gb,
                                                      const double* __restrict__ pa,
                                                      double* __restrict__ pb,
                                                      const double* __restrict__ te_in,
                                                      double* __restrict__ te_out, Macro m,
-                                                     Consts c, Geom gm, int write_all) {
+                                                     Consts c, Geom gm, TileGrid tg,
+                                                     int write_all) {
   const int nx = gm.nx, ny = gm.ny;
-  const int i = blockIdx.x * TX + threadIdx.x + 1;
-  const int j = blockIdx.y * TY + threadIdx.y + 1;
-  if (i > nx || j > ny) return;
+  int i, j;
+  if (!tile_node(tg, gm, &i, &j)) return;
   const int64_t P = gm.pitch;
   const int64_t q = (int64_t)j * P + i;
   const bool w_edge = (i == 1), e_edge = (i == nx), s_edge = (j == 1), n_edge = (j == ny);
@@ -303,8 +304,9 @@ inline dim3 full_grid(const Geom& g) {
 
 hipError_t launch_fused_step32(bgk_handle* h, int step_parity, bool write_all) {
   const Geom& g = h->geom;
-  dim3 blk(TX, TY);
-  dim3 grd((unsigned)((g.nx + TX - 1) / TX), (unsigned)((g.ny + TY - 1) / TY));
+  const TileGrid tg = make_tile_grid(g);
+  dim3 blk(TILE_W);
+  dim3 grd((unsigned)(tg.per * NXCD));
   Macro m{h->rho, h->u, h->v, h->T};
   const double* te_in = h->teast[step_parity & 1];
   double* te_out = h->teast[(step_parity + 1) & 1];
@@ -312,11 +314,11 @@ hipError_t launch_fused_step32(bgk_handle* h, int step_parity, bool write_all) {
   if (h->prm.fp32_collision)
     hipLaunchKernelGGL(k_fused32<float>, grd, blk, 0, h->stream, pops32(h->f32[a]),
                        pops32(h->g32[a]), pops32(h->f32[b]), pops32(h->g32[b]), h->phi[h->pc],
-                       h->phi[1 - h->pc], te_in, te_out, m, h->c, g, write_all ? 1 : 0);
+                       h->phi[1 - h->pc], te_in, te_out, m, h->c, g, tg, write_all ? 1 : 0);
   else
     hipLaunchKernelGGL(k_fused32<double>, grd, blk, 0, h->stream, pops32(h->f32[a]),
                        pops32(h->g32[a]), pops32(h->f32[b]), pops32(h->g32[b]), h->phi[h->pc],
-                       h->phi[1 - h->pc], te_in, te_out, m, h->c, g, write_all ? 1 : 0);
+                       h->phi[1 - h->pc], te_in, te_out, m, h->c, g, tg, write_all ? 1 : 0);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) {
     h->cur = b;
