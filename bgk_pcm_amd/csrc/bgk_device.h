@@ -56,34 +56,34 @@ struct Consts {
 // tiles of one row j and TILE_W consecutive columns.  gfx950 dispatches
 // workgroup b to XCD b % 8, and each XCD has its own L2; a pull reads one
 // cache line past each end of its row segment, which the neighbouring
-// segment also reads.  So workgroup b takes tile (b % 8) * per + b / 8: each
-// XCD walks a contiguous range of rows, segment after segment, and the
-// shared lines are L2 hits.  The grid is padded to a multiple of 8
-// workgroups; padding workgroups return at once.
+// segment of the same row also reads.  So all segments of a row go to one
+// XCD (workgroup b serves row (b/8 / nseg)*8 + b%8 + 1, segment b/8 % nseg):
+// the shared lines are L2 hits, and at any moment the eight XCDs stream
+// eight neighbouring rows.  Workgroups past the last row return at once.
 constexpr int NXCD = 8;
 constexpr int TILE_W = 256;
 
 struct TileGrid {
   int nseg;        // tiles per row
   int64_t ntiles;  // nseg * ny
-  int64_t per;     // tiles per XCD (grid / 8)
+  int64_t per;     // workgroups per XCD: ceil(ny/8) * nseg
 };
 
 inline TileGrid make_tile_grid(const Geom& g) {
   TileGrid t;
   t.nseg = (g.nx + TILE_W - 1) / TILE_W;
   t.ntiles = (int64_t)t.nseg * g.ny;
-  t.per = (t.ntiles + NXCD - 1) / NXCD;
+  t.per = (int64_t)((g.ny + NXCD - 1) / NXCD) * t.nseg;
   return t;
 }
 
 // Node of this thread; false for threads of padding workgroups or past Nx.
 __device__ __forceinline__ bool tile_node(const TileGrid& t, const Geom& g, int* i, int* j) {
   const int64_t b = blockIdx.x;
-  const int64_t tile = (b % NXCD) * t.per + b / NXCD;
-  if (tile >= t.ntiles) return false;
-  *j = (int)(tile / t.nseg) + 1;
-  *i = (int)(tile % t.nseg) * TILE_W + (int)threadIdx.x + 1;
+  const int64_t k = b / NXCD;
+  *j = (int)((k / t.nseg) * NXCD + (b % NXCD)) + 1;
+  if (*j > g.ny) return false;
+  *i = (int)(k % t.nseg) * TILE_W + (int)threadIdx.x + 1;
   return *i <= g.nx;
 }
 

@@ -19,13 +19,21 @@ Each fixture holds:
               window steps
   fields      names of the F stored fields (a subset of the 46)
   text_<fn>   text files written by the reference diagnostics
+
+driver_cav24x20.npz instead holds every file the complete reference program
+(oracle/_ref/step7/bgk_thermal, its own main.f90) writes for a 24x20,
+2000-step input, as text: file_<name> per file, the input.dat records in
+`params`, and the deterministic stdout lines in `stdout_check`.
 """
 from __future__ import annotations
 
 import hashlib
 import json
+import os
 import pathlib
+import subprocess
 import sys
+import tempfile
 from dataclasses import asdict
 
 import numpy as np
@@ -35,7 +43,8 @@ sys.path.insert(0, str(HERE.parents[1]))
 sys.path.insert(0, str(HERE.parent))
 
 from bgk_pcm_amd.params import Params  # noqa: E402
-from oracle_py import FIELD_NAMES, run_ref  # noqa: E402
+from bgk_pcm_amd.params import write_input_dat  # noqa: E402
+from oracle_py import FIELD_NAMES, REF_DIR, run_ref  # noqa: E402
 
 STATE = [f"f{k}" for k in range(9)] + [f"g{k}" for k in range(9)] + ["rho", "u", "v", "T", "phi"]
 ALL = list(FIELD_NAMES)
@@ -58,6 +67,30 @@ CASES = {
     "run_cav60x16_u0_400": (Params(nx=60, ny=16, re=5.0, length=16.0), 400, [], [], STATE, 100),
     "run_clean_45x7_300": (Params(nx=45, ny=7, rounding=1), 300, [], [], STATE, 0),
 }
+
+
+DRIVER_CASE = ("driver_cav24x20", Params(nx=24, ny=20, iter=2000, noutput=2))
+
+
+def driver_fixture():
+    name, p = DRIVER_CASE
+    exe = REF_DIR / "step7" / "bgk_thermal"
+    with tempfile.TemporaryDirectory(prefix="bgkdrv_") as wd:
+        write_input_dat(p, os.path.join(wd, "input.dat"))
+        r = subprocess.run([str(exe)], cwd=wd, capture_output=True, text=True, timeout=600,
+                           check=True)
+        data = {"params": np.array(json.dumps(asdict(p)))}
+        for fn in sorted(os.listdir(wd)):
+            if fn == "input.dat":
+                continue
+            with open(os.path.join(wd, fn)) as fh:
+                data["file_" + fn] = np.array(fh.read())
+        data["stdout_check"] = np.array(
+            "\n".join(l for l in r.stdout.splitlines() if "CHECK" in l))
+    path = HERE / f"{name}.npz"
+    np.savez_compressed(path, **data)
+    print(f"{name}: {path.stat().st_size / 1024:.0f} KiB")
+    return path
 
 
 def main() -> None:
@@ -88,6 +121,8 @@ def main() -> None:
         digest = hashlib.sha256(path.read_bytes()).hexdigest()
         manifest.append(f"{digest}  {path.name}")
         print(f"{name}: {path.stat().st_size / 1024:.0f} KiB")
+    path = driver_fixture()
+    manifest.append(f"{hashlib.sha256(path.read_bytes()).hexdigest()}  {path.name}")
     (HERE / "MANIFEST.sha256").write_text("\n".join(manifest) + "\n")
 
 

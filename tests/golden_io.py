@@ -17,7 +17,8 @@ INTERIOR_ONLY = {f"fp{k}" for k in range(9)} | {f"gp{k}" for k in range(9)} | {
 
 
 def fixtures():
-    return sorted(p.stem for p in GOLDEN.glob("*.npz"))
+    """State fixtures (the driver fixture, driver_*.npz, holds files only)."""
+    return sorted(p.stem for p in GOLDEN.glob("*.npz") if not p.stem.startswith("driver_"))
 
 
 class Golden:

@@ -1,0 +1,194 @@
+"""Checks of the oracle that the mathematics fixes, independent of the
+reference build: each would fail for a plausible slip in the restatement
+(a dropped term, a wrong sign or index, a swapped operand).
+
+ - equilibria: at rest f_k = w_k*rho exactly; zeroth and first moments of
+   the equilibrium are rho and rho*u;
+ - streaming is the pull permutation fp_k(x) = f_k(x - c_k) (checked with
+   numpy rolls on random data);
+ - regularized collision conserves density exactly in exact arithmetic:
+   sum_k f_k' = rho to rounding in the clean policy, to FP32 rounding of
+   the nine temporaries in the STEP_7 policy;
+ - phase field: phi = +1 and phi = -1 everywhere are fixed points for any
+   temperature; with T far below the melting temperature (tanh = -1) a
+   solid node cannot melt;
+ - forces: in a solid node (phi = -1) with tau_f = 1 and rho = 1 the drag
+   removes the velocity exactly; with phi = +1 and grav = 0 nothing changes;
+ - moments are the averages of the pre- and post-collision moments;
+ - field_check / mass_check / diag sums equal numpy's;
+ - the 32-bit Ny**3 of diag.f90 wraps to 0 at Ny = 2048;
+ - melting is monotone in a conduction-only run (physics sanity).
+"""
+import numpy as np
+import pytest
+
+from bgk_pcm_amd.params import Params
+from oracle_py import Oracle
+
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+
+
+def random_state(o, rng, scale=1e-3):
+    """Fill populations with a perturbed equilibrium-like random state."""
+    ny, nx = o.ny + 2, o.nx + 2
+    rho = 1.0 + 0.02 * rng.standard_normal((ny, nx))
+    T = 278.0 + 3.0 * rng.standard_normal((ny, nx))
+    for k in range(9):
+        o.field(f"f{k}")[...] = W[k] * rho * (1 + scale * rng.standard_normal((ny, nx)))
+        o.field(f"g{k}")[...] = W[k] * T * (1 + scale * rng.standard_normal((ny, nx)))
+
+
+def test_rest_equilibrium_is_exact():
+    o = Oracle(Params(nx=30, ny=20, re=0.0))
+    rho = o.field("rho")
+    for k in range(9):
+        assert np.array_equal(o.field(f"f{k}"), W[k] * rho * 1.0)
+        assert np.array_equal(o.field(f"g{k}"), W[k] * o.field("T") * 1.0)
+
+
+def test_equilibrium_moments():
+    p = Params(nx=30, ny=20, re=40.0, length=20.0)  # u0 = Re*nu/L = 0.33
+    o = Oracle(p)
+    f = np.stack([o.field(f"f{k}") for k in range(9)])
+    rho, u = o.field("rho"), o.field("u")
+    inner = (slice(1, -1), slice(1, -1))
+    assert np.allclose(f.sum(0)[inner], rho[inner], rtol=1e-15, atol=0)
+    jx = np.tensordot(CX, f, axes=1)
+    assert np.allclose(jx[inner], (rho * u)[inner], rtol=1e-14, atol=1e-17)
+    assert u[5, 5] != 0.0
+
+
+def test_streaming_is_the_pull_permutation():
+    rng = np.random.default_rng(3)
+    o = Oracle(Params(nx=17, ny=13), initialize=False)
+    random_state(o, rng)
+    o.stage("streaming")
+    for k in range(9):
+        src = o.field(f"f{k}")
+        # value at (j, i) after the pull is the source at (j - cy, i - cx)
+        rolled = np.roll(np.roll(src, CY[k], axis=0), CX[k], axis=1)
+        assert np.array_equal(o.field(f"fp{k}")[1:-1, 1:-1], rolled[1:-1, 1:-1]), k
+
+
+@pytest.mark.parametrize("rounding,tol", [(1, 2e-15), (0, 1e-7)])
+def test_collision_conserves_density(rounding, tol):
+    rng = np.random.default_rng(5)
+    o = Oracle(Params(nx=24, ny=18, rounding=rounding), initialize=False)
+    random_state(o, rng)
+    o.field("phi")[...] = rng.uniform(-1, 1, o.field("phi").shape)
+    for s in ("streaming", "moments_pre", "phase_field", "forces", "collision"):
+        o.stage(s)
+    f = np.stack([o.field(f"f{k}") for k in range(9)])
+    g = np.stack([o.field(f"g{k}") for k in range(9)])
+    inner = (slice(1, -1), slice(1, -1))
+    assert np.allclose(f.sum(0)[inner], o.field("rho")[inner], rtol=tol, atol=0)
+    # the temperature populations conserve the (latent-heat corrected) T
+    assert np.allclose(g.sum(0)[inner], o.field("T")[inner], rtol=tol, atol=0)
+
+
+@pytest.mark.parametrize("phase", [1.0, -1.0])
+def test_uniform_phase_is_a_fixed_point(phase):
+    rng = np.random.default_rng(11)
+    o = Oracle(Params(nx=20, ny=20), initialize=False)
+    o.field("phi")[...] = phase
+    o.field("T")[...] = rng.uniform(200.0, 350.0, o.field("T").shape)
+    o.stage("phase_field")
+    assert np.all(o.field("phi") == phase)
+
+
+def test_cold_solid_cannot_melt():
+    o = Oracle(Params(nx=20, ny=20), initialize=False)
+    rng = np.random.default_rng(2)
+    phi0 = rng.choice([-1.0, 1.0], size=o.field("phi").shape)
+    o.field("phi")[...] = phi0
+    o.field("T")[...] = 0.0  # tanh((0-273)/10) == -1 in double
+    o.stage("phase_field")
+    phi = o.field("phi")[1:-1, 1:-1]
+    solid = phi0[1:-1, 1:-1] < 0
+    assert np.all(phi[solid] == -1.0)
+    assert np.all(phi <= phi0[1:-1, 1:-1])
+
+
+def test_drag_stops_a_solid_node_and_spares_a_liquid_one():
+    p = Params(nx=10, ny=8, tau_f=1.0, grav=0.0)
+    o = Oracle(p, initialize=False)
+    rng = np.random.default_rng(4)
+    o.field("rho")[...] = 1.0
+    o.field("T")[...] = 280.0
+    u0 = rng.standard_normal(o.field("u").shape) * 1e-3
+    v0 = rng.standard_normal(o.field("v").shape) * 1e-3
+    o.field("u")[...] = u0
+    o.field("v")[...] = v0
+    phi = np.full(o.field("phi").shape, -1.0)
+    phi[:, 5:] = 1.0
+    o.field("phi")[...] = phi
+    o.field("phi_prec")[...] = phi
+    o.stage("forces")
+    u, v = o.field("u")[1:-1, 1:-1], o.field("v")[1:-1, 1:-1]
+    solid = phi[1:-1, 1:-1] < 0
+    assert np.all(u[solid] == 0.0) and np.all(v[solid] == 0.0)
+    assert np.array_equal(u[~solid], u0[1:-1, 1:-1][~solid])
+    assert np.array_equal(v[~solid], v0[1:-1, 1:-1][~solid])
+    assert np.array_equal(o.field("T"), np.full_like(u0, 280.0))  # phi == phi_prec
+
+
+def test_moments_average_pre_and_post_collision():
+    rng = np.random.default_rng(8)
+    o = Oracle(Params(nx=16, ny=12, rounding=1), initialize=False)
+    random_state(o, rng)
+    for s in ("streaming", "moments_pre"):
+        o.stage(s)
+    T2, u2 = o.field("T2").copy(), o.field("u2").copy()
+    for s in ("phase_field", "forces", "collision", "moments"):
+        o.stage(s)
+    f = np.stack([o.field(f"f{k}") for k in range(9)])
+    g = np.stack([o.field(f"g{k}") for k in range(9)])
+    inner = (slice(1, -1), slice(1, -1))
+    rho = f.sum(0)
+    assert np.allclose(o.field("rho")[inner], rho[inner], rtol=1e-15)
+    assert np.allclose(o.field("T")[inner], ((g.sum(0) + T2) / 2)[inner], rtol=1e-15)
+    ux = np.tensordot(CX, f, axes=1) / rho
+    assert np.allclose(o.field("u")[inner], ((ux + u2) / 2)[inner], rtol=1e-12, atol=1e-18)
+
+
+def test_reductions_equal_numpy():
+    p = Params(nx=40, ny=30)
+    o = Oracle(p)
+    o.step(300)
+    inner = (slice(1, -1), slice(1, -1))
+    rho, T, phi = (o.field(f)[inner] for f in ("rho", "T", "phi"))
+    np.testing.assert_allclose(o.field_check(), [rho.mean(), T.mean(), phi.mean()], rtol=1e-14)
+    mass = (p.nx * p.ny * p.rho0 - rho.sum()) * 100 / (p.nx * p.ny * p.rho0)
+    got = o.mass_check()
+    assert abs(got[0] - mass) < 1e-9
+    np.testing.assert_allclose(got[1:], [rho.mean(), T.mean()], rtol=1e-14)
+    s = o.diag_sums()
+    Tf = o.field("T")
+    dT = p.t0 - p.t_in_sol
+    want0 = np.sum(p.ny * ((p.t0 - Tf[1:-1, 2]) / 2) / dT)
+    want1 = np.sum(p.ny * np.abs((-3 * Tf[1:-1, 1] + 4 * Tf[1:-1, 2] - Tf[1:-1, 3]) / 2) / dT)
+    np.testing.assert_allclose(s[:2], [want0, want1], rtol=1e-13)
+    assert s[2] == np.count_nonzero(phi > 0)
+    ii = np.arange(1, p.nx + 1)[None, :].repeat(p.ny, 0)
+    assert s[3] == ii[phi == 0].sum()
+
+
+def test_grashof_wraps_like_default_integer():
+    o = Oracle(Params(nx=8, ny=2048), initialize=True)
+    out, _ = o.diag(1000)
+    assert out[8] == 0.0  # Gras: 2048**3 = 2**33 wraps to 0 in 32 bits
+    o2 = Oracle(Params(nx=8, ny=1024), initialize=True)
+    assert o2.diag(1000)[0][8] > 0.0
+
+
+def test_melting_is_monotone_without_convection():
+    p = Params(nx=64, ny=8, grav=0.0)
+    o = Oracle(p)
+    melted = []
+    for _ in range(6):
+        o.step(400)
+        melted.append(np.count_nonzero(o.field("phi")[1:-1, 1:-1] > 0))
+    assert all(b >= a for a, b in zip(melted, melted[1:]))
+    assert melted[-1] > melted[0] > 0
