@@ -82,12 +82,14 @@ size_t carve(bgk_handle* h, char* base) {
   for (int b = 0; b < 2; ++b)
     for (int k = 0; k < 9; ++k) {
       if (fp32) {
-        h->f32[b][k] = cv.plane<float>(n);
-        h->g32[b][k] = cv.plane<float>(n);
+        float* pr = reinterpret_cast<float*>(cv.plane<float2>(n));
+        h->f32[b][k] = pr;
+        h->g32[b][k] = pr ? pr + 1 : nullptr;
         h->f[b][k] = h->g[b][k] = nullptr;
       } else {
-        h->f[b][k] = cv.plane<double>(n);
-        h->g[b][k] = cv.plane<double>(n);
+        double* pr = reinterpret_cast<double*>(cv.plane<double2>(n));
+        h->f[b][k] = pr;
+        h->g[b][k] = pr ? pr + 1 : nullptr;
         h->f32[b][k] = h->g32[b][k] = nullptr;
       }
     }
@@ -99,14 +101,13 @@ size_t carve(bgk_handle* h, char* base) {
   h->T = cv.plane<double>(n);
   if (fp32) {
     h->rho2 = h->u2 = h->v2 = h->T2 = nullptr;
-    h->tmp = cv.plane<double>(n);
   } else {
     h->rho2 = cv.plane<double>(n);
     h->u2 = cv.plane<double>(n);
     h->v2 = cv.plane<double>(n);
     h->T2 = cv.plane<double>(n);
-    h->tmp = nullptr;
   }
+  h->tmp = cv.plane<double>(n);
   h->teast[0] = cv.block<double>(h->ny + 2);
   h->teast[1] = cv.block<double>(h->ny + 2);
   h->partial = cv.block<double>(bgk::reduce_partial_count(h));
@@ -165,9 +166,10 @@ int launched(hipError_t e, const char* what) {
 // Device plane of an FP64 field, or null with *fp32 set for the FP32
 // populations (k in 0..8, base w_k*rho0 or w_k*T0), or null on error.
 struct FieldRef {
-  double* d64 = nullptr;
-  float* d32 = nullptr;
+  double* d64 = nullptr;  // FP64 field; element stride `stride`
+  float* d32 = nullptr;   // FP32 deviation member of a pair plane (stride 2)
   double base = 0.0;
+  int stride = 1;
 };
 
 int resolve(bgk_handle* h, int field, FieldRef* r) {
@@ -181,6 +183,7 @@ int resolve(bgk_handle* h, int field, FieldRef* r) {
       r->base = isf ? h->c.w[k] * h->c.rho0 : h->c.w[k] * h->c.t0;
     } else {
       r->d64 = isf ? h->f[h->cur][k] : h->g[h->cur][k];
+      r->stride = 2;
     }
     return BGK_OK;
   }
@@ -189,6 +192,7 @@ int resolve(bgk_handle* h, int field, FieldRef* r) {
     const bool isf = field < BGK_G0;
     const int k = isf ? field - BGK_FP0 : field - BGK_GP0;
     r->d64 = isf ? h->f[1 - h->cur][k] : h->g[1 - h->cur][k];
+    r->stride = 2;
     return BGK_OK;
   }
   switch (field) {
@@ -337,12 +341,14 @@ int bgk_upload(bgk_handle* h, int32_t field, const double* src, int64_t ld) {
   if (ld < h->nx + 2) return fail(BGK_E_INVALID, "ld=%lld < Nx+2=%d", (long long)ld, h->nx + 2);
   FieldRef r;
   if ((rc = resolve(h, field, &r))) return rc;
-  double* dst = r.d64 ? r.d64 : h->tmp;
+  const bool direct = r.d64 && r.stride == 1;
+  double* dst = direct ? r.d64 : h->tmp;
   HIP_TRY(hipMemcpy2DAsync(dst, (size_t)h->pitch * sizeof(double), src, (size_t)ld * sizeof(double),
                            (size_t)(h->nx + 2) * sizeof(double), (size_t)(h->ny + 2),
                            hipMemcpyHostToDevice, h->stream),
           "hipMemcpy2DAsync(H2D)");
-  if (r.d32) HIP_TRY(bgk::launch_f64_to_dev(h, h->tmp, r.d32, r.base), "f64_to_dev");
+  if (r.d32) HIP_TRY(bgk::launch_f64_to_dev(h, h->tmp, r.d32, 2, r.base), "f64_to_dev");
+  if (!direct && r.d64) HIP_TRY(bgk::launch_copy64(h, h->tmp, 1, r.d64, r.stride), "copy64");
   HIP_TRY(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
   return BGK_OK;
 }
@@ -356,7 +362,10 @@ int bgk_download(bgk_handle* h, int32_t field, double* dst, int64_t ld) {
   if ((rc = resolve(h, field, &r))) return rc;
   const double* src = r.d64;
   if (r.d32) {
-    HIP_TRY(bgk::launch_dev_to_f64(h, r.d32, h->tmp, r.base), "dev_to_f64");
+    HIP_TRY(bgk::launch_dev_to_f64(h, r.d32, 2, h->tmp, r.base), "dev_to_f64");
+    src = h->tmp;
+  } else if (r.stride != 1) {
+    HIP_TRY(bgk::launch_copy64(h, r.d64, r.stride, h->tmp, 1), "copy64");
     src = h->tmp;
   }
   HIP_TRY(hipMemcpy2DAsync(dst, (size_t)ld * sizeof(double), src, (size_t)h->pitch * sizeof(double),
@@ -462,7 +471,10 @@ int bgk_gather(bgk_handle* h, int32_t field, int32_t n, const int32_t* i, const 
                   h->ny + 1);
   const double* plane = r.d64;
   if (r.d32) {
-    HIP_TRY(bgk::launch_dev_to_f64(h, r.d32, h->tmp, r.base), "dev_to_f64");
+    HIP_TRY(bgk::launch_dev_to_f64(h, r.d32, 2, h->tmp, r.base), "dev_to_f64");
+    plane = h->tmp;
+  } else if (r.stride != 1) {
+    HIP_TRY(bgk::launch_copy64(h, r.d64, r.stride, h->tmp, 1), "copy64");
     plane = h->tmp;
   }
   // small buffer: first half outputs, second half (i,j) pairs

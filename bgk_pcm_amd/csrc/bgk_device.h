@@ -28,11 +28,30 @@ struct Geom {
   int64_t pitch;
 };
 
+// Populations are stored as (f_k, g_k) pairs: plane k of a buffer holds,
+// for node q, f_k at element 2q and g_k at element 2q+1.  Both are pulled
+// from the same neighbour, so one 2-wide load serves both, and a step
+// streams 9 population planes in and 9 out instead of 18 and 18.
+// Strided2 is the view of one member of the pair (base = plane for f,
+// plane + 1 for g).
+template <typename T>
+struct Strided2 {
+  T* base;
+  __host__ __device__ __forceinline__ T& operator[](int64_t q) const { return base[2 * q]; }
+};
+
 struct Pops64 {
-  double* p[9];
+  Strided2<double> p[9];
 };
 struct Pops32 {
-  float* p[9];
+  Strided2<float> p[9];
+};
+// Pair views of the same planes (used by the fused kernels).
+struct Pairs64 {
+  double2* p[9];
+};
+struct Pairs32 {
+  float2* p[9];
 };
 
 // Scalars of module `shared` the step reads (STEP_7/setup.f90:79-105,

@@ -34,9 +34,10 @@ struct Macro {
   double *rho, *u, *v, *T;
 };
 
-inline Pops64 pops(double* const* p) {
-  Pops64 r;
-  for (int k = 0; k < 9; ++k) r.p[k] = p[k];
+// pair view of buffer b: plane k holds (f_k, g_k) per node
+inline Pairs64 pairs(double* const* fplanes) {
+  Pairs64 r;
+  for (int k = 0; k < 9; ++k) r.p[k] = reinterpret_cast<double2*>(fplanes[k]);
   return r;
 }
 
@@ -47,7 +48,7 @@ __global__ void k_teast_prologue(const double* __restrict__ T, double* __restric
 }
 
 template <bool STEP7>
-__global__ __launch_bounds__(TILE_W) void k_fused64(Pops64 fa, Pops64 ga, Pops64 fb, Pops64 gb,
+__global__ __launch_bounds__(TILE_W) void k_fused64(Pairs64 A, Pairs64 B,
                                                      const double* __restrict__ pa,
                                                      double* __restrict__ pb,
                                                      const double* __restrict__ te_in,
@@ -67,9 +68,9 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pops64 fa, Pops64 ga, Pops64
   if (!edge) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const int64_t s = q - CX[k] - (int64_t)CY[k] * P;
-      fp[k] = fa.p[k][s];
-      gp[k] = ga.p[k][s];
+      const double2 fg = A.p[k][q - CX[k] - (int64_t)CY[k] * P];
+      fp[k] = fg.x;
+      gp[k] = fg.y;
     }
   } else {
 #pragma unroll
@@ -77,15 +78,15 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pops64 fa, Pops64 ga, Pops64
       const int si = i - CX[k];
       const int sj = j - CY[k];
       if (si >= 1 && si <= nx && sj >= 1 && sj <= ny) {
-        fp[k] = fa.p[k][(int64_t)sj * P + si];
+        fp[k] = A.p[k][(int64_t)sj * P + si].x;
       } else if ((k == 6 && si == nx + 1 && sj == 0) || (k == 7 && si == nx + 1 && sj == ny + 1)) {
-        fp[k] = fa.p[k][(int64_t)sj * P + si];  // never written by BC_f
+        fp[k] = A.p[k][(int64_t)sj * P + si].x;  // never written by BC_f
       } else {
-        fp[k] = fa.p[OPP[k]][q];  // bounce-back
+        fp[k] = A.p[OPP[k]][q].x;  // bounce-back
       }
       const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
       if (si >= 1 && si <= nx) {
-        gp[k] = ga.p[k][(int64_t)sjc * P + si];
+        gp[k] = A.p[k][(int64_t)sjc * P + si].y;
       } else if (si == 0) {
         const int64_t h = (int64_t)sjc * P;
         gp[k] = equilibrium_k(c, k, c.tin, m.u[h], m.v[h]);
@@ -131,10 +132,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pops64 fa, Pops64 ga, Pops64
   regularised_bgk<STEP7>(c, c.omega_f, rho, u, v, fp, fo);
   regularised_bgk<STEP7>(c, c.omega_g, T, u, v, gp, go);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    fb.p[k][q] = fo[k];
-    gb.p[k][q] = go[k];
-  }
+  for (int k = 0; k < 9; ++k) B.p[k][q] = make_double2(fo[k], go[k]);
   pb[q] = phi;
 
   // ---- moments (STEP_7/moments.f90:11-25), where somebody reads them
@@ -185,13 +183,13 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pops64 fa, Pops64 ga, Pops64
         if (s_edge) {
           m.T[nx + 1] = te_in[1];
           pb[nx + 1] = pa[P + nx + 1];
-          fb.p[6][nx + 1] = fa.p[6][nx + 1];
+          B.p[6][nx + 1].x = A.p[6][nx + 1].x;
         }
         if (n_edge) {
           const int64_t hn = (int64_t)(ny + 1) * P + nx + 1;
           m.T[hn] = te_in[ny];
           pb[hn] = pa[(int64_t)ny * P + nx + 1];
-          fb.p[7][hn] = fa.p[7][hn];
+          B.p[7][hn].x = A.p[7][hn].x;
         }
       }
     }
@@ -221,12 +219,10 @@ hipError_t launch_fused_step64(bgk_handle* h, int step_parity, bool write_all) {
   double* te_out = h->teast[(step_parity + 1) & 1];
   const int a = h->cur, b = 1 - h->cur;
   if (h->prm.rounding == BGK_ROUND_STEP7)
-    hipLaunchKernelGGL(k_fused64<true>, grd, blk, 0, h->stream, pops(h->f[a]), pops(h->g[a]),
-                       pops(h->f[b]), pops(h->g[b]), h->phi[h->pc], h->phi[1 - h->pc], te_in,
+    hipLaunchKernelGGL(k_fused64<true>, grd, blk, 0, h->stream, pairs(h->f[a]), pairs(h->f[b]), h->phi[h->pc], h->phi[1 - h->pc], te_in,
                        te_out, m, h->c, g, tg, write_all ? 1 : 0);
   else
-    hipLaunchKernelGGL(k_fused64<false>, grd, blk, 0, h->stream, pops(h->f[a]), pops(h->g[a]),
-                       pops(h->f[b]), pops(h->g[b]), h->phi[h->pc], h->phi[1 - h->pc], te_in,
+    hipLaunchKernelGGL(k_fused64<false>, grd, blk, 0, h->stream, pairs(h->f[a]), pairs(h->f[b]), h->phi[h->pc], h->phi[1 - h->pc], te_in,
                        te_out, m, h->c, g, tg, write_all ? 1 : 0);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) {

@@ -29,7 +29,14 @@ struct Macro {
 
 inline Pops32 pops32(float* const* p) {
   Pops32 r;
-  for (int k = 0; k < 9; ++k) r.p[k] = p[k];
+  for (int k = 0; k < 9; ++k) r.p[k].base = p[k];
+  return r;
+}
+
+// pair view of buffer b: plane k holds (f_k - w_k rho0, g_k - w_k T0) per node
+inline Pairs32 pairs32(float* const* fplanes) {
+  Pairs32 r;
+  for (int k = 0; k < 9; ++k) r.p[k] = reinterpret_cast<float2*>(fplanes[k]);
   return r;
 }
 
@@ -88,7 +95,7 @@ __device__ __forceinline__ double ghost_dev(const Consts& c, int k, double m, do
 }
 
 template <typename RC>
-__global__ __launch_bounds__(TILE_W) void k_fused32(Pops32 fa, Pops32 ga, Pops32 fb, Pops32 gb,
+__global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
                                                      const double* __restrict__ pa,
                                                      double* __restrict__ pb,
                                                      const double* __restrict__ te_in,
@@ -107,9 +114,9 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pops32 fa, Pops32 ga, Pops32
   if (!edge) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const int64_t s = q - CX[k] - (int64_t)CY[k] * P;
-      df[k] = (double)fa.p[k][s];
-      dg[k] = (double)ga.p[k][s];
+      const float2 fg = A.p[k][q - CX[k] - (int64_t)CY[k] * P];
+      df[k] = (double)fg.x;
+      dg[k] = (double)fg.y;
     }
   } else {
 #pragma unroll
@@ -117,15 +124,15 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pops32 fa, Pops32 ga, Pops32
       const int si = i - CX[k];
       const int sj = j - CY[k];
       if (si >= 1 && si <= nx && sj >= 1 && sj <= ny) {
-        df[k] = (double)fa.p[k][(int64_t)sj * P + si];
+        df[k] = (double)A.p[k][(int64_t)sj * P + si].x;
       } else if ((k == 6 && si == nx + 1 && sj == 0) || (k == 7 && si == nx + 1 && sj == ny + 1)) {
-        df[k] = (double)fa.p[k][(int64_t)sj * P + si];
+        df[k] = (double)A.p[k][(int64_t)sj * P + si].x;
       } else {
-        df[k] = (double)fa.p[OPP[k]][q];
+        df[k] = (double)A.p[OPP[k]][q].x;
       }
       const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
       if (si >= 1 && si <= nx) {
-        dg[k] = (double)ga.p[k][(int64_t)sjc * P + si];
+        dg[k] = (double)A.p[k][(int64_t)sjc * P + si].y;
       } else if (si == 0) {
         const int64_t h = (int64_t)sjc * P;
         dg[k] = ghost_dev(c, k, c.tin, c.t0, m.u[h], m.v[h]);
@@ -178,8 +185,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pops32 fa, Pops32 ga, Pops32
   for (int k = 0; k < 9; ++k) {
     fs[k] = (float)fo[k];
     gs[k] = (float)go[k];
-    fb.p[k][q] = fs[k];
-    gb.p[k][q] = gs[k];
+    B.p[k][q] = make_float2(fs[k], gs[k]);
   }
   pb[q] = phi;
 
@@ -236,13 +242,13 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pops32 fa, Pops32 ga, Pops32
         if (s_edge) {
           m.T[nx + 1] = te_in[1];
           pb[nx + 1] = pa[P + nx + 1];
-          fb.p[6][nx + 1] = fa.p[6][nx + 1];
+          B.p[6][nx + 1].x = A.p[6][nx + 1].x;
         }
         if (n_edge) {
           const int64_t hn = (int64_t)(ny + 1) * P + nx + 1;
           m.T[hn] = te_in[ny];
           pb[hn] = pa[(int64_t)ny * P + nx + 1];
-          fb.p[7][hn] = fa.p[7][hn];
+          B.p[7][hn].x = A.p[7][hn].x;
         }
       }
     }
@@ -253,23 +259,33 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pops32 fa, Pops32 ga, Pops32
   }
 }
 
-// Whole-plane conversions between FP64 values and FP32 deviations.
-__global__ void k_dev_to_f64(const float* __restrict__ src, double* __restrict__ dst, double base,
-                             Geom gm) {
+// Whole-plane conversions between an FP64 plane and one member of a pair
+// plane (stride 2): FP32 deviations or FP64 values.
+__global__ void k_dev_to_f64(const float* __restrict__ src, int sstride, double* __restrict__ dst,
+                             double base, Geom gm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > gm.nx + 1 || j > gm.ny + 1) return;
   const int64_t q = (int64_t)j * gm.pitch + i;
-  dst[q] = base + (double)src[q];
+  dst[q] = base + (double)src[q * sstride];
 }
 
-__global__ void k_f64_to_dev(const double* __restrict__ src, float* __restrict__ dst, double base,
-                             Geom gm) {
+__global__ void k_f64_to_dev(const double* __restrict__ src, float* __restrict__ dst, int dstride,
+                             double base, Geom gm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > gm.nx + 1 || j > gm.ny + 1) return;
   const int64_t q = (int64_t)j * gm.pitch + i;
-  dst[q] = (float)(src[q] - base);
+  dst[q * dstride] = (float)(src[q] - base);
+}
+
+__global__ void k_copy64(const double* __restrict__ src, int sstride, double* __restrict__ dst,
+                         int dstride, Geom gm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > gm.nx + 1 || j > gm.ny + 1) return;
+  const int64_t q = (int64_t)j * gm.pitch + i;
+  dst[q * dstride] = src[q * sstride];
 }
 
 // STEP_7/initialization.f90:15-152 for an FP32 handle: FP64 equilibria,
@@ -312,12 +328,12 @@ hipError_t launch_fused_step32(bgk_handle* h, int step_parity, bool write_all) {
   double* te_out = h->teast[(step_parity + 1) & 1];
   const int a = h->cur, b = 1 - h->cur;
   if (h->prm.fp32_collision)
-    hipLaunchKernelGGL(k_fused32<float>, grd, blk, 0, h->stream, pops32(h->f32[a]),
-                       pops32(h->g32[a]), pops32(h->f32[b]), pops32(h->g32[b]), h->phi[h->pc],
+    hipLaunchKernelGGL(k_fused32<float>, grd, blk, 0, h->stream, pairs32(h->f32[a]),
+                       pairs32(h->f32[b]), h->phi[h->pc],
                        h->phi[1 - h->pc], te_in, te_out, m, h->c, g, tg, write_all ? 1 : 0);
   else
-    hipLaunchKernelGGL(k_fused32<double>, grd, blk, 0, h->stream, pops32(h->f32[a]),
-                       pops32(h->g32[a]), pops32(h->f32[b]), pops32(h->g32[b]), h->phi[h->pc],
+    hipLaunchKernelGGL(k_fused32<double>, grd, blk, 0, h->stream, pairs32(h->f32[a]),
+                       pairs32(h->f32[b]), h->phi[h->pc],
                        h->phi[1 - h->pc], te_in, te_out, m, h->c, g, tg, write_all ? 1 : 0);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) {
@@ -327,15 +343,23 @@ hipError_t launch_fused_step32(bgk_handle* h, int step_parity, bool write_all) {
   return e;
 }
 
-hipError_t launch_dev_to_f64(bgk_handle* h, const float* src, double* dst, double base) {
-  hipLaunchKernelGGL(k_dev_to_f64, full_grid(h->geom), dim3(TX, TY), 0, h->stream, src, dst, base,
-                     h->geom);
+hipError_t launch_dev_to_f64(bgk_handle* h, const float* src, int sstride, double* dst,
+                             double base) {
+  hipLaunchKernelGGL(k_dev_to_f64, full_grid(h->geom), dim3(TX, TY), 0, h->stream, src, sstride,
+                     dst, base, h->geom);
   return hipGetLastError();
 }
 
-hipError_t launch_f64_to_dev(bgk_handle* h, const double* src, float* dst, double base) {
-  hipLaunchKernelGGL(k_f64_to_dev, full_grid(h->geom), dim3(TX, TY), 0, h->stream, src, dst, base,
-                     h->geom);
+hipError_t launch_f64_to_dev(bgk_handle* h, const double* src, float* dst, int dstride,
+                             double base) {
+  hipLaunchKernelGGL(k_f64_to_dev, full_grid(h->geom), dim3(TX, TY), 0, h->stream, src, dst,
+                     dstride, base, h->geom);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy64(bgk_handle* h, const double* src, int sstride, double* dst, int dstride) {
+  hipLaunchKernelGGL(k_copy64, full_grid(h->geom), dim3(TX, TY), 0, h->stream, src, sstride, dst,
+                     dstride, h->geom);
   return hipGetLastError();
 }
 

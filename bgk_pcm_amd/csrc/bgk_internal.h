@@ -10,21 +10,23 @@
 
 // One lattice resident in HBM.
 //
-// Layout: every field is a plane of (Ny+2) rows of `pitch` elements; node
-// (i,j) of a plane is at plane[j*pitch + i].  The pitch is a multiple of
-// 128 bytes and each plane starts one element before a 128-byte boundary,
-// so the first interior node of every row (i=1) is 128-byte aligned.  All
-// planes come from one allocation (`pool`).
+// Layout: every field is a plane of (Ny+2) rows of `pitch` nodes; node
+// (i,j) of a plane is at plane[j*pitch + i] (pair planes: elements
+// 2*(j*pitch+i) and +1).  The pitch is a multiple of 32 nodes and each
+// plane starts one node before a 256-byte boundary, so node (1,j) of every
+// row starts a cache line.  All planes come from one allocation (`pool`).
 //
 // Populations are double-buffered: f[cur] is the reference's f (post
 // collision) and f[1-cur] its fp (post streaming); the fused step reads
-// f[cur], writes f[1-cur] and flips cur.  phi[pc] is phi and phi[1-pc]
-// phi_prec.  FP32 handles hold f32/g32 = f - w*rho0, g - w*T0 instead.
+// f[cur], writes f[1-cur] and flips cur.  f_k and g_k of a buffer share one
+// pair plane (f at element 2q, g at 2q+1): f[b][k] points at the plane and
+// g[b][k] = f[b][k] + 1.  phi[pc] is phi and phi[1-pc] phi_prec.  FP32
+// handles hold f32/g32 = f - w*rho0, g - w*T0 the same way.
 struct bgk_handle {
   bgk_params prm;
   int nx, ny;
   int64_t pitch;        // elements per row
-  int64_t plane_elems;  // elements between consecutive planes (>= (ny+2)*pitch)
+  int64_t plane_elems;  // nodes per plane: (ny+2)*pitch
   int device;
   hipStream_t own_stream;
   hipStream_t stream;
@@ -44,7 +46,7 @@ struct bgk_handle {
   int pc;
   double *rho, *u, *v, *T, *rho2, *u2, *v2, *T2;
   double* teast[2];  // averaged T(Nx, 0..Ny+1) of the previous step, ping-pong
-  double* tmp;       // one FP64 plane: staging for FP32 conversions
+  double* tmp;       // one FP64 plane: staging for population transfers
   double* partial;   // reduction partials
   double* small;     // small device outputs (reductions, gathers)
   int64_t small_elems;
@@ -75,9 +77,12 @@ hipError_t launch_teast_prologue(bgk_handle* h);
 hipError_t launch_fused_step64(bgk_handle* h, int step_parity, bool write_all);
 hipError_t launch_fused_step32(bgk_handle* h, int step_parity, bool write_all);
 
-// ---- FP32 conversions (bgk_fused32.hip)
-hipError_t launch_dev_to_f64(bgk_handle* h, const float* src, double* dst, double base);
-hipError_t launch_f64_to_dev(bgk_handle* h, const double* src, float* dst, double base);
+// ---- plane conversions (bgk_fused32.hip); strides in elements
+hipError_t launch_dev_to_f64(bgk_handle* h, const float* src, int sstride, double* dst,
+                             double base);
+hipError_t launch_f64_to_dev(bgk_handle* h, const double* src, float* dst, int dstride,
+                             double base);
+hipError_t launch_copy64(bgk_handle* h, const double* src, int sstride, double* dst, int dstride);
 // initialization of an FP32 handle (FP64 equilibria rounded to deviations)
 hipError_t launch_initialization32(bgk_handle* h);
 

@@ -21,7 +21,7 @@ inline Macro macro_of(bgk_handle* h) {
 
 inline Pops64 pops(double* const* p) {
   Pops64 r;
-  for (int k = 0; k < 9; ++k) r.p[k] = p[k];
+  for (int k = 0; k < 9; ++k) r.p[k].base = p[k];
   return r;
 }
 
@@ -68,7 +68,7 @@ __global__ void k_bc_f_we(Pops64 F, Macro m, Consts c, Geom gm, int wbc, int ebc
   if (j > ny + 1) return;
   const int64_t P = gm.pitch;
   auto at = [&](int i, int jj) { return (int64_t)jj * P + i; };
-  double* const* f = F.p;
+  const Strided2<double>* f = F.p;
   const bool inner = (j >= 1 && j <= ny);
   double uu, vv, uv, upv, umv;
   // WEST
@@ -142,7 +142,7 @@ __global__ void k_bc_f_ns(Pops64 F, Macro m, Consts c, Geom gm, int nbc, int sbc
   if (i > nx + 1) return;
   const int64_t P = gm.pitch;
   auto at = [&](int ii, int jj) { return (int64_t)jj * P + ii; };
-  double* const* f = F.p;
+  const Strided2<double>* f = F.p;
   const bool inner = (i >= 1 && i <= nx);
   // NORTH
   if (nbc == 0) {
@@ -203,7 +203,7 @@ __global__ void k_bc_g_we(Pops64 G, Macro m, Consts c, Geom gm) {
   if (j > ny) return;
   const int64_t P = gm.pitch;
   auto at = [&](int ii, int jj) { return (int64_t)jj * P + ii; };
-  double* const* g = G.p;
+  const Strided2<double>* g = G.p;
   {
     const double uw = m.u[at(0, j)], vw = m.v[at(0, j)];
     const double uu = uw * uw, vv = vw * vw, uv = 1.5 * (uu + vv), upv = uw + vw, umv = uw - vw;

@@ -9,7 +9,10 @@ input.dat, must write the same files with the same bytes: the VTK files
 print every field with 17 significant digits, so this is bit-for-bit parity
 of rho, u, v, T and phi (halo included) at the output steps.  The one
 exception allowed is fort.64's reductions, compared at their printed
-precision, which is also exact here.
+precision, which is also exact here.  Nusselt.dat is compared by
+structure only: the reference prints its Nu arrays without ever computing
+them (they are set only under DEBUG_GA, STEP_7/moments.f90:39-63), i.e.
+uninitialized memory (the fixture holds a NaN), while the driver prints 0.
 """
 import json
 import os
@@ -38,11 +41,25 @@ def test_driver_writes_the_reference_files(tmp_path, mode):
                        timeout=240)
     assert r.returncode == 0, r.stderr[-2000:]
     want_files = {k[5:]: str(d[k]) for k in d.files if k.startswith("file_")}
+    failures = []
     for fn, want in want_files.items():
         path = tmp_path / fn
-        assert path.exists(), fn
+        if not path.exists():
+            failures.append(f"{fn}: missing")
+            continue
         got = path.read_text()
-        assert got == want, f"{fn} differs"
+        gl, wl = got.splitlines(), want.splitlines()
+        if fn == "Nusselt.dat":
+            ok = len(gl) == len(wl) and all(a.split()[0] == b.split()[0] for a, b in zip(gl, wl))
+            if not ok:
+                failures.append(f"{fn}: structure differs")
+            continue
+        if got != want:
+            k = next((n for n, (a, b) in enumerate(zip(gl, wl)) if a != b), min(len(gl), len(wl)))
+            failures.append(f"{fn} differs at line {k + 1} ({len(gl)} vs {len(wl)} lines):\n"
+                            f"  got:  {gl[k] if k < len(gl) else '<eof>'}\n"
+                            f"  want: {wl[k] if k < len(wl) else '<eof>'}")
+    assert not failures, "\n".join(failures)
     checks = [l for l in r.stdout.splitlines() if "CHECK" in l]
     assert "\n".join(checks) == str(d["stdout_check"])
     assert "INFO: Mlups" in r.stdout
