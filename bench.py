@@ -14,8 +14,10 @@ Printed JSON line (one):
                over the fused kernel's mean launch time, measured with HIP
                events on the stream the kernel runs on; vs 8 TB/s
   fp64         the same for the FP64 fused kernel (304 B per update)
-  fp32_error   relative max-norm error of the FP32-storage run against the
-               FP64 run (STEP_7 rounding), per field, after --error-steps
+  fp32_error   relative max-norm error (per field, liquid fraction for phi)
+               after --error-steps of the FP32-storage run against the FP64
+               run with clean FP64 temporaries and against the STEP_7-rounding
+               FP64 run, and of the STEP_7 run against the clean one
   cpu_baseline the reference's own STEP_OPENMP time step (compiled from its
                sources into oracle/_ref) timed on this host's cores on a
                bounded sample of the same workload; the CPU oracle (a port)
@@ -97,23 +99,37 @@ def gpu_run(p, stream, steps, warmup):
     return kms, wms, fc
 
 
-def fp32_error(p64, p32, steps: int):
-    """Relative max-norm error of the FP32-storage run (p32) against the
-    FP64 run (p64) after `steps` steps from the same initial state."""
+def rel_errors(a, b):
+    """Relative max-norm error max|a-b|/max|b| per field over the interior,
+    with the liquid fraction (1+phi)/2 in place of phi."""
     import numpy as np
-    from bgk_pcm_amd.lattice import Lattice
-    from bgk_pcm_amd.params import FP32_DEV, FP64
-    assert p64.precision == FP64 and p32.precision == FP32_DEV
     out = {}
-    with Lattice(p64) as a, Lattice(p32) as b:
-        a.step(steps)
-        b.step(steps)
-        for f in FIELDS:
-            x, y = b.download(f)[1:-1, 1:-1], a.download(f)[1:-1, 1:-1]
-            if f == "phi":  # liquid fraction
-                x, y = (1 + x) / 2, (1 + y) / 2
-            out[f] = float(np.abs(x - y).max() / np.abs(y).max())
+    for f in FIELDS:
+        x, y = a[f][1:-1, 1:-1], b[f][1:-1, 1:-1]
+        if f == "phi":
+            x, y = (1 + x) / 2, (1 + y) / 2
+        out[f] = float(np.abs(x - y).max() / np.abs(y).max())
     return out
+
+
+def fp32_error(p64, p32, steps: int):
+    """Errors after `steps` steps from the same initial state: the
+    FP32-storage run against the FP64 run with clean FP64 temporaries and
+    against the STEP_7-rounding FP64 run, and the STEP_7 run against the
+    clean one (the reference's own FP32-temporary noise floor)."""
+    from bgk_pcm_amd.lattice import Lattice
+    from bgk_pcm_amd.params import FP32_DEV, FP64, ROUND_CLEAN, ROUND_STEP7
+    assert p64.precision == FP64 and p32.precision == FP32_DEV
+    runs = {}
+    for name, p in (("fp64_step7", p64.replace(rounding=ROUND_STEP7)),
+                    ("fp64_clean", p64.replace(rounding=ROUND_CLEAN)), ("fp32", p32)):
+        with Lattice(p) as lat:
+            lat.step(steps)
+            runs[name] = {f: lat.download(f) for f in FIELDS}
+    return {"steps": steps,
+            "fp32_vs_fp64_clean": rel_errors(runs["fp32"], runs["fp64_clean"]),
+            "fp32_vs_fp64_step7": rel_errors(runs["fp32"], runs["fp64_step7"]),
+            "fp64_step7_vs_fp64_clean": rel_errors(runs["fp64_step7"], runs["fp64_clean"])}
 
 
 def cpu_baseline(p, steps: int):
@@ -218,8 +234,7 @@ def main():
                        "traffic": traffic_from_profile(n, "fp64")}
         res["fp32_over_fp64"] = round(res["value"] / res["fp64"]["mlups"], 3)
     if args.error_steps > 0:
-        res["fp32_error"] = {"steps": args.error_steps, "vs": "FP64 run, STEP_7 rounding",
-                             "rel_max_norm": fp32_error(p, p32, args.error_steps)}
+        res["fp32_error"] = fp32_error(p, p32, args.error_steps)
     if args.calibrate_copy > 0:
         nb = args.calibrate_copy * 1024 * 1024 // 8
         a = torch.ones(nb, dtype=torch.float64, device="cuda")

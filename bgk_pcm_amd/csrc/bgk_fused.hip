@@ -68,7 +68,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pairs64 A, Pairs64 B,
   if (!edge) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const double2 fg = A.p[k][q - CX[k] - (int64_t)CY[k] * P];
+      const double2 fg = stream_load(&A.p[k][q - CX[k] - (int64_t)CY[k] * P]);
       fp[k] = fg.x;
       gp[k] = fg.y;
     }
@@ -132,8 +132,8 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pairs64 A, Pairs64 B,
   regularised_bgk<STEP7>(c, c.omega_f, rho, u, v, fp, fo);
   regularised_bgk<STEP7>(c, c.omega_g, T, u, v, gp, go);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) B.p[k][q] = make_double2(fo[k], go[k]);
-  pb[q] = phi;
+  for (int k = 0; k < 9; ++k) stream_store(&B.p[k][q], fo[k], go[k]);
+  stream_store(&pb[q], phi);
 
   // ---- moments (STEP_7/moments.f90:11-25), where somebody reads them
   if (edge || write_all) {

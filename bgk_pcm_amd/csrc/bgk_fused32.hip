@@ -114,7 +114,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
   if (!edge) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const float2 fg = A.p[k][q - CX[k] - (int64_t)CY[k] * P];
+      const float2 fg = stream_load(&A.p[k][q - CX[k] - (int64_t)CY[k] * P]);
       df[k] = (double)fg.x;
       dg[k] = (double)fg.y;
     }
@@ -185,9 +185,9 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
   for (int k = 0; k < 9; ++k) {
     fs[k] = (float)fo[k];
     gs[k] = (float)go[k];
-    B.p[k][q] = make_float2(fs[k], gs[k]);
+    stream_store(&B.p[k][q], fs[k], gs[k]);
   }
-  pb[q] = phi;
+  stream_store(&pb[q], phi);
 
   // ---- moments of the stored state, FP64
   if (edge || write_all) {
