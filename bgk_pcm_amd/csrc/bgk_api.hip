@@ -150,6 +150,7 @@ void set_constants(bgk_handle* h) {
   c.beta = (p.rounding == BGK_ROUND_STEP7) ? (double)(float)beta : beta;
   c.grav = p.grav;
   c.t_width = p.t_width;
+  c.g_ref = 273.0;  // T_crit of STEP_7/phase_field.f90:16
 }
 
 int check_handle(bgk_handle* h) {
@@ -188,7 +189,7 @@ int resolve(bgk_handle* h, int field, FieldRef* r) {
     const int k = isf ? field - BGK_F0 : field - BGK_G0;
     if (fp32) {
       r->d32 = isf ? h->f32[h->cur][k] : h->g32[h->cur][k];
-      r->base = isf ? h->c.w[k] * h->c.rho0 : h->c.w[k] * h->c.t0;
+      r->base = isf ? h->c.w[k] * h->c.rho0 : h->c.w[k] * h->c.g_ref;
     } else {
       r->d64 = isf ? h->f[h->cur][k] : h->g[h->cur][k];
       r->stride = 2;

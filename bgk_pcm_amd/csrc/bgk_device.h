@@ -99,6 +99,7 @@ struct Consts {
   double beta;      // (1 - rho1/rho0)/delta_T, FP32-rounded under STEP7 rounding
   double grav;
   double t_width;
+  double g_ref;     // FP32 storage: g is stored as g_k - w_k*g_ref (g_ref = 273)
 };
 
 // XCD-aware tiling of the fused kernels.  The lattice interior is cut into

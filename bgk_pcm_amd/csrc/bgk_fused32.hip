@@ -1,19 +1,23 @@
 // bgk_fused32.hip -- the fused time step with FP32 population storage.
 //
-// Storage: f_k - w_k*rho0 and g_k - w_k*T0 in FP32 (the deviations from the
-// rest equilibrium are ~1e-3 of the populations, so FP32 keeps ~30x more
-// absolute precision than storing f and g themselves in FP32).  phi, the
-// east-wall temperature column and all macroscopic fields stay FP64.
+// Storage: f_k - w_k*rho0 and g_k - w_k*T_crit in FP32, deviations from the
+// rest equilibrium at the solid density and at the melting temperature
+// (T_crit = 273, STEP_7/phase_field.f90:16).  They are ~1e-2 of the
+// populations, so FP32 keeps far more absolute precision than storing f and
+// g themselves, and the temperature is most precise where the phase field
+// is sensitive to it: at the melting front, where g_k - w_k*T_crit ~ 0.  phi,
+// the east-wall temperature column and all macroscopic fields stay FP64.
 //
 // Arithmetic: moments, temperature, phase field, latent heat and forces are
 // FP64.  The collision runs in `RC` (double by default): on gfx950 unpacked
 // FP32 and FP64 FMA issue at the same rate, so FP64 costs registers, not
 // time, while FP32 collision arithmetic loses accuracy at the melting front.
 // Same folding of BC_f/BC_g and the same halo bookkeeping as the FP64 kernel
-// (bgk_fused.hip); the algebra is the reference's written on deviations:
-//   feq_k - w_k rho0 = w_k (drho + rho P_k(u,v)),
+// (bgk_fused.hip); the algebra is the reference's written on deviations
+// (m0 = rho0 or T_crit, dm = m - m0):
+//   feq_k - w_k m0 = w_k (dm + m P_k(u,v)),
 //   P_k = 3 c_k.u + 4.5 (c_k.u)^2 - 1.5 u.u,
-//   f_k' - w_k rho0 = (1-omega) (ft_k - w_k rho0) + omega (feq_k - w_k rho0).
+//   f_k' - w_k m0 = (1-omega) (ft_k - w_k m0) + omega (feq_k - w_k m0).
 // The FP32 rounding of ft/gt done by STEP_7 (implicit REAL) is not modelled.
 #include "bgk_internal.h"
 
@@ -135,10 +139,10 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
         dg[k] = (double)A.p[k][(int64_t)sjc * P + si].y;
       } else if (si == 0) {
         const int64_t h = (int64_t)sjc * P;
-        dg[k] = ghost_dev(c, k, c.tin, c.t0, m.u[h], m.v[h]);
+        dg[k] = ghost_dev(c, k, c.tin, c.g_ref, m.u[h], m.v[h]);
       } else {
         const int64_t h = (int64_t)sjc * P + nx + 1;
-        dg[k] = ghost_dev(c, k, te_in[sjc], c.t0, m.u[h], m.v[h]);
+        dg[k] = ghost_dev(c, k, te_in[sjc], c.g_ref, m.u[h], m.v[h]);
       }
     }
   }
@@ -157,12 +161,12 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
   const int64_t r0 = (int64_t)j * P;
   const double tap[9] = {pa[q],          pa[r0 + i + 1], pa[rn + i],     pa[r0 + i - 1], pa[rs + i],
                          pa[rn + i + 1], pa[rn + i - 1], pa[rs + i - 1], pa[rs + i + 1]};
-  const double phi = phase_update(c, c.t0 + dT, tap);
+  const double phi = phase_update(c, c.g_ref + dT, tap);
   const double pc = tap[0];
 
   // ---- latent heat and forces, FP64
   dT = dT - (c.dt_st * (phi - pc) / 2.0);
-  const double T = c.t0 + dT;
+  const double T = c.g_ref + dT;
   {
     double fy = rho * c.grav * c.beta * T * (1.0 + phi / 2.0);
     const double fx = -(u * (1.0 - phi) * 0.5);
@@ -202,7 +206,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
     const double dtt = b[0] + b[1] + b[2] + b[3] + b[4] + b[5] + b[6] + b[7] + b[8];
     const double tu = (a[1] - a[3] + a[5] - a[6] - a[7] + a[8]) / r;
     const double tv = (a[2] - a[4] + a[5] + a[6] - a[7] - a[8]) / r;
-    const double Tav = c.t0 + (dtt + dT2) * 0.5;
+    const double Tav = c.g_ref + (dtt + dT2) * 0.5;
     const double uav = (tu + u2) * 0.5;
     const double vav = (tv + v2) * 0.5;
     if (edge) {
@@ -303,7 +307,7 @@ __global__ void k_initialization32(Pops32 f, Pops32 g, Macro m, double* phi, dou
   for (int k = 0; k < 9; ++k) f.p[k][q] = (float)(eq[k] - c.w[k] * c.rho0);
   equilibria(c, s0.T, s0.u, s0.v, eq);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) g.p[k][q] = (float)(eq[k] - c.w[k] * c.t0);
+  for (int k = 0; k < 9; ++k) g.p[k][q] = (float)(eq[k] - c.w[k] * c.g_ref);
   m.rho[q] = s0.rho;
   m.u[q] = s0.u;
   m.v[q] = s0.v;

@@ -21,7 +21,7 @@
 // f[cur], writes f[1-cur] and flips cur.  f_k and g_k of a buffer share one
 // pair plane (f at element 2q, g at 2q+1): f[b][k] points at the plane and
 // g[b][k] = f[b][k] + 1.  phi[pc] is phi and phi[1-pc] phi_prec.  FP32
-// handles hold f32/g32 = f - w*rho0, g - w*T0 the same way.
+// handles hold f32/g32 = f - w*rho0, g - w*273 (T_crit) the same way.
 struct bgk_handle {
   bgk_params prm;
   int nx, ny;

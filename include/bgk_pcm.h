@@ -47,8 +47,8 @@ enum {
 /* storage precision of the populations (bgk_params.precision) */
 enum {
   BGK_FP64 = 0,     /* f, g stored and computed in FP64                        */
-  BGK_FP32_DEV = 1  /* f - w*rho0 and g - w*T0 stored in FP32; phase field,
-                       enthalpy update and moments in FP64                     */
+  BGK_FP32_DEV = 1  /* f - w*rho0 and g - w*T_crit (T_crit = 273) stored in
+                       FP32; phase field, enthalpy update and moments in FP64  */
 };
 
 /* rounding policy of the FP64 arithmetic (bgk_params.rounding) */

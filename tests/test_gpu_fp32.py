@@ -1,4 +1,4 @@
-"""FP32 population storage (f - w*rho0, g - w*T0 in FP32) against the FP64
+"""FP32 population storage (f - w*rho0, g - w*T_crit in FP32) against the FP64
 oracle.
 
 Error measure: relative max-norm max|a - b| / max|b| over the interior, per
@@ -67,7 +67,7 @@ def test_fp32_initial_populations():
     o = Oracle(p)
     with Lattice(p.replace(precision=FP32_DEV)) as lat:
         for k in range(9):
-            for name, m0 in ((f"f{k}", p.rho0), (f"g{k}", p.t0)):
+            for name, m0 in ((f"f{k}", p.rho0), (f"g{k}", 273.0)):
                 a, b = lat.download(name), o.field(name)
                 # only the FP32 rounding of the deviation from w*m0
                 w = (4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36)[k]
