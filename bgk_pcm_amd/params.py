@@ -151,6 +151,22 @@ class Params:
             raise ValueError("rounding must be ROUND_STEP7 or ROUND_CLEAN")
 
 
+def rayleigh(p: Params) -> float:
+    """Rayleigh number as STEP_7/diag.f90:31-40 defines it,
+    Ra = Pr * grav * beta_gras * Delta_T * Ny**3 / nu**2 with
+    beta_gras = (rho0/rho1 - 1)/Delta_T, Pr = (2 tau_f - 1)/(2 tau_g - 1),
+    nu = (tau_f - 1/2)/3, evaluated in floating point (the reference's
+    32-bit Ny**3 wraps for Ny > 1290)."""
+    nu = (p.tau_f - 0.5) / 3.0
+    pr = (2.0 * p.tau_f - 1.0) / (2.0 * p.tau_g - 1.0)
+    return pr * p.grav * (p.rho0 / p.rho1 - 1.0) * float(p.ny) ** 3 / nu ** 2
+
+
+def grav_for_rayleigh(p: Params, ra: float) -> float:
+    """grav that gives Rayleigh number `ra` (definition of `rayleigh`)."""
+    return ra / rayleigh(p.replace(grav=1.0))
+
+
 _FORTRAN_REAL = re.compile(r"^[+-]?(\d+\.?\d*|\.\d+)([eEdDqQ][+-]?\d+)?$")
 _FORTRAN_INT = re.compile(r"^[+-]?\d+$")
 
