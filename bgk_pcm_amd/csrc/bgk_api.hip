@@ -8,6 +8,7 @@
 #include <sys/resource.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -86,28 +87,40 @@ struct Carver {
 size_t carve(bgk_handle* h, char* base) {
   Carver cv{base, 0};
   const int64_t n = h->plane_elems;
-  const bool fp32 = h->prm.precision == BGK_FP32_DEV;
-  for (int b = 0; b < 2; ++b)
+  const int prec = h->prm.precision;
+  const bool fp32 = prec == BGK_FP32_DEV, mom = prec == BGK_FP32_MOM;
+  for (int b = 0; b < 2; ++b) {
+    h->ma[b] = nullptr;
+    h->mb[b] = h->mc[b] = nullptr;
+    for (int k = 0; k < 9; ++k) {
+      h->f[b][k] = h->g[b][k] = nullptr;
+      h->f32[b][k] = h->g32[b][k] = nullptr;
+    }
+    if (mom) {
+      h->ma[b] = cv.plane<double2>(n);
+      h->mb[b] = cv.plane<float4>(n);
+      h->mc[b] = cv.plane<float4>(n);
+      continue;
+    }
     for (int k = 0; k < 9; ++k) {
       if (fp32) {
         float* pr = reinterpret_cast<float*>(cv.plane<float2>(n));
         h->f32[b][k] = pr;
         h->g32[b][k] = pr ? pr + 1 : nullptr;
-        h->f[b][k] = h->g[b][k] = nullptr;
       } else {
         double* pr = reinterpret_cast<double*>(cv.plane<double2>(n));
         h->f[b][k] = pr;
         h->g[b][k] = pr ? pr + 1 : nullptr;
-        h->f32[b][k] = h->g32[b][k] = nullptr;
       }
     }
+  }
   h->phi[0] = cv.plane<double>(n);
   h->phi[1] = cv.plane<double>(n);
   h->rho = cv.plane<double>(n);
   h->u = cv.plane<double>(n);
   h->v = cv.plane<double>(n);
   h->T = cv.plane<double>(n);
-  if (fp32) {
+  if (fp32 || mom) {
     h->rho2 = h->u2 = h->v2 = h->T2 = nullptr;
   } else {
     h->rho2 = cv.plane<double>(n);
@@ -134,6 +147,7 @@ void set_constants(bgk_handle* h) {
   for (int k = 0; k < 9; ++k) c.coef[k] = (0.5 * c.w[k]) / (c.cs2 * c.cs2);
   c.omega_f = 1.0 / p.tau_f;
   c.omega_g = 1.0 / p.tau_g;
+  c.inv_omega_f = 1.0 / c.omega_f;
   c.rho0 = p.rho0;
   c.rho1 = p.rho1;
   c.t0 = p.t0;
@@ -150,7 +164,13 @@ void set_constants(bgk_handle* h) {
   c.beta = (p.rounding == BGK_ROUND_STEP7) ? (double)(float)beta : beta;
   c.grav = p.grav;
   c.t_width = p.t_width;
+  c.two_over_tw = 2.0 / p.t_width;
   c.g_ref = 273.0;  // T_crit of STEP_7/phase_field.f90:16
+  // The initial equilibria of the two ghosts BC_f never writes
+  // (initialization.f90:22-26,76-81: rho0 right of the interface, u = v = 0
+  // on rows 0 and Ny+1), in the operand order of collision.f90:23-41.
+  c.frozen6 = c.w[6] * p.rho0 * (1.0 - 0.0);
+  c.frozen7 = c.w[7] * p.rho0 * (1.0 - 0.0);
 }
 
 int check_handle(bgk_handle* h) {
@@ -162,8 +182,8 @@ int check_handle(bgk_handle* h) {
 
 int need_fp64(bgk_handle* h, const char* what) {
   if (h->prm.precision != BGK_FP64)
-    return fail(BGK_E_UNSUPPORTED, "%s needs an FP64 handle (this one stores FP32 deviations)",
-                what);
+    return fail(BGK_E_UNSUPPORTED, "%s needs an FP64 handle (this one stores %s)", what,
+                h->prm.precision == BGK_FP32_MOM ? "FP32 moments" : "FP32 deviations");
   return BGK_OK;
 }
 
@@ -184,6 +204,8 @@ struct FieldRef {
 int resolve(bgk_handle* h, int field, FieldRef* r) {
   const bool fp32 = h->prm.precision == BGK_FP32_DEV;
   if (field < 0 || field >= BGK_NFIELDS) return fail(BGK_E_INVALID, "field id %d", field);
+  if (h->prm.precision == BGK_FP32_MOM && field < BGK_RHO)
+    return fail(BGK_E_UNSUPPORTED, "a moment handle stores no populations (field %d)", field);
   if (field < BGK_FP0 || (field >= BGK_G0 && field < BGK_GP0)) {
     const bool isf = field < BGK_FP0;
     const int k = isf ? field - BGK_F0 : field - BGK_G0;
@@ -212,7 +234,8 @@ int resolve(bgk_handle* h, int field, FieldRef* r) {
     case BGK_PHI: r->d64 = h->phi[h->pc]; break;
     case BGK_PHI_PREC: r->d64 = h->phi[1 - h->pc]; break;
     default:
-      if (fp32) return fail(BGK_E_UNSUPPORTED, "rho2/u2/v2/T2 exist only on FP64 handles");
+      if (h->prm.precision != BGK_FP64)
+        return fail(BGK_E_UNSUPPORTED, "rho2/u2/v2/T2 exist only on FP64 handles");
       r->d64 = field == BGK_RHO2 ? h->rho2 : field == BGK_U2 ? h->u2 : field == BGK_V2 ? h->v2 : h->T2;
   }
   return BGK_OK;
@@ -255,12 +278,12 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
       p->rho0 == 0.0 || p->t0 == p->t_in_sol)
     return fail(BGK_E_INVALID, "tau_f, tau_g, stefan_number, T_width, rho0 and T0-T_in_sol "
                                "must be nonzero");
-  if (p->precision != BGK_FP64 && p->precision != BGK_FP32_DEV)
+  if (p->precision != BGK_FP64 && p->precision != BGK_FP32_DEV && p->precision != BGK_FP32_MOM)
     return fail(BGK_E_INVALID, "precision %d", p->precision);
   if (p->rounding != BGK_ROUND_STEP7 && p->rounding != BGK_ROUND_CLEAN)
     return fail(BGK_E_INVALID, "rounding %d", p->rounding);
   const bool fused_ok = p->west_bc == 4 && p->east_bc == 4 && p->north_bc == 4 && p->south_bc == 4;
-  if (p->precision == BGK_FP32_DEV && !fused_ok)
+  if (p->precision != BGK_FP64 && !fused_ok)
     return fail(BGK_E_UNSUPPORTED, "FP32 storage is implemented for the no-slip cavity "
                                    "(all four boundary types 4) only");
   int ndev = 0;
@@ -284,6 +307,14 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
   h->fused_ok = fused_ok;
   h->cur = 0;
   h->pc = 0;
+  // tiling of the moment kernel: BGK_MOM_ROWS (rows per strip, default 32)
+  // and BGK_MOM_VARIANT (1..8: workgroup width 64/128/256/512 with a 4- or
+  // 3-row ring; 0: chosen from Nx) override the defaults; tuning knobs,
+  // every value is correct
+  h->strip_rows = 0;
+  h->mom_variant = 0;
+  if (const char* env = std::getenv("BGK_MOM_ROWS")) h->strip_rows = std::max(0, std::atoi(env));
+  if (const char* env = std::getenv("BGK_MOM_VARIANT")) h->mom_variant = std::max(0, std::atoi(env));
   set_constants(h);
 
   h->pool_bytes = carve(h, nullptr);
@@ -340,6 +371,8 @@ int bgk_initialization(bgk_handle* h) {
   int rc = check_handle(h);
   if (rc) return rc;
   if (h->prm.precision == BGK_FP32_DEV) return launched(bgk::launch_initialization32(h), "initialization");
+  if (h->prm.precision == BGK_FP32_MOM)
+    return launched(bgk::launch_initialization_mom(h), "initialization");
   return launched(bgk::launch_initialization(h), "initialization");
 }
 
@@ -348,6 +381,8 @@ int bgk_upload(bgk_handle* h, int32_t field, const double* src, int64_t ld) {
   if (rc) return rc;
   if (!src) return fail(BGK_E_INVALID, "null source");
   if (ld < h->nx + 2) return fail(BGK_E_INVALID, "ld=%lld < Nx+2=%d", (long long)ld, h->nx + 2);
+  if (h->prm.precision == BGK_FP32_MOM)
+    return fail(BGK_E_UNSUPPORTED, "a moment handle's state is set by bgk_initialization only");
   FieldRef r;
   if ((rc = resolve(h, field, &r))) return rc;
   const bool direct = r.d64 && r.stride == 1;
@@ -401,10 +436,12 @@ int bgk_step(bgk_handle* h, int32_t nsteps) {
   if (nsteps == 0) return BGK_OK;
   if (h->fused_ok) {
     if ((rc = launched(bgk::launch_teast_prologue(h), "teast prologue"))) return rc;
-    const bool fp32 = h->prm.precision == BGK_FP32_DEV;
+    const int prec = h->prm.precision;
     for (int s = 0; s < nsteps; ++s) {
       const bool last = (s == nsteps - 1);
-      hipError_t e = fp32 ? bgk::launch_fused_step32(h, s, last) : bgk::launch_fused_step64(h, s, last);
+      hipError_t e = prec == BGK_FP32_DEV   ? bgk::launch_fused_step32(h, s, last)
+                     : prec == BGK_FP32_MOM ? bgk::launch_fused_step_mom(h, s, last)
+                                            : bgk::launch_fused_step64(h, s, last);
       if (e != hipSuccess) return hip_fail(e, "fused step");
     }
     return BGK_OK;

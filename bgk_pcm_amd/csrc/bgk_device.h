@@ -100,6 +100,11 @@ struct Consts {
   double grav;
   double t_width;
   double g_ref;     // FP32 storage: g is stored as g_k - w_k*g_ref (g_ref = 273)
+  // f6(Nx+1,0) and f7(Nx+1,Ny+1): ghosts no BC_f branch writes, frozen at
+  // their initial equilibria (moment storage keeps them as constants)
+  double frozen6, frozen7;
+  double inv_omega_f;  // 1/omega_f, for kernels that need not round as STEP_7
+  double two_over_tw;  // 2/T_width, same
 };
 
 // XCD-aware tiling of the fused kernels.  The lattice interior is cut into
@@ -264,11 +269,6 @@ __device__ __forceinline__ NodeInit initial_node(const Consts& c, const Geom& gm
 // Phase-field update of one node, STEP_7/phase_field.f90:24-52.  tap[] are
 // phi_prec at C, E, N, W, S, NE, NW, SW, SE; returns the new phi.
 __device__ __forceinline__ double phase_update(const Consts& c, double T, const double tap[9]) {
-  // T_act = 0 and T_crit = 273 (phase_field.f90:15-16): both reaction rates
-  // take tanh of the same argument.
-  const double th = tanh_fdlibm(((T - 273.0) - 0.0) / c.t_width);
-  const double react_plus = 0.5 * (1.0 + th);
-  const double react_minus = 0.5 * (1.0 - th);
   double a = 0.5 * (1.0 + tap[0]);
   double kp = a * a;
   double b = 0.5 * (1.0 - tap[0]);
@@ -283,9 +283,20 @@ __device__ __forceinline__ double phase_update(const Consts& c, double T, const 
     b = 0.5 * (1.0 - tap[m]);
     km = km + b * b;
   }
+  const double pc = tap[0];
+  // Inside one phase (all nine taps -1, or all +1) each reaction term is
+  // multiplied by an exact zero (kp = 0 or 1 + pc = 0, km = 0 or 1 - pc = 0)
+  // and the update returns pc bit for bit; the tanh is needed only near the
+  // front.  (With a non-finite T the reference would give NaN here too; T
+  // itself carries the NaN either way.)
+  if ((kp == 0.0 || 1.0 - pc == 0.0) && (km == 0.0 || 1.0 + pc == 0.0)) return pc;
+  // T_act = 0 and T_crit = 273 (phase_field.f90:15-16): both reaction rates
+  // take tanh of the same argument.
+  const double th = tanh_fdlibm(((T - 273.0) - 0.0) / c.t_width);
+  const double react_plus = 0.5 * (1.0 + th);
+  const double react_minus = 0.5 * (1.0 - th);
   kp = (kp / 8.0) * react_plus;
   km = (km / 8.0) * react_minus;
-  const double pc = tap[0];
   return pc + (0.5 * kp * (1.0 - pc) - 0.5 * km * (1.0 + pc));
 }
 

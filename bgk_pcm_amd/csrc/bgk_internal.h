@@ -12,9 +12,10 @@
 //
 // Layout: every field is a plane of (Ny+2) rows of `pitch` nodes; node
 // (i,j) of a plane is at plane[j*pitch + i] (pair planes: elements
-// 2*(j*pitch+i) and +1).  The pitch is a multiple of 32 nodes and each
-// plane starts one node before a 256-byte boundary, so node (1,j) of every
-// row starts a cache line.  All planes come from one allocation (`pool`).
+// 2*(j*pitch+i) and +1).  The pitch is a multiple of 64 nodes and not of
+// 256, and each plane starts one node before a 256-byte boundary, so node
+// (1,j) of every row starts a cache line.  All planes come from one
+// allocation (`pool`).
 //
 // Populations are double-buffered: f[cur] is the reference's f (post
 // collision) and f[1-cur] its fp (post streaming); the fused step reads
@@ -22,6 +23,8 @@
 // pair plane (f at element 2q, g at 2q+1): f[b][k] points at the plane and
 // g[b][k] = f[b][k] + 1.  phi[pc] is phi and phi[1-pc] phi_prec.  FP32
 // handles hold f32/g32 = f - w*rho0, g - w*273 (T_crit) the same way.
+// Moment handles hold per node ma = (rho, T - 273) FP64, mb = (u, v, Pf_xx,
+// Pf_yy) and mc = (Pf_xy, Pg_xx, Pg_yy, Pg_xy) FP32, double-buffered by cur.
 struct bgk_handle {
   bgk_params prm;
   int nx, ny;
@@ -40,6 +43,12 @@ struct bgk_handle {
   // FP32 deviation state (FP32 handles)
   float* f32[2][9];
   float* g32[2][9];
+  // FP32 moment state (moment handles)
+  double2* ma[2];
+  float4* mb[2];
+  float4* mc[2];
+  int strip_rows;   // rows per workgroup strip of the moment kernel (0: default)
+  int mom_variant;  // tiling variant of the moment kernel (0: default)
   int cur;
 
   double* phi[2];
@@ -76,6 +85,8 @@ hipError_t launch_teast_prologue(bgk_handle* h);
 // moments of every node (else only the boundary-adjacent ones).
 hipError_t launch_fused_step64(bgk_handle* h, int step_parity, bool write_all);
 hipError_t launch_fused_step32(bgk_handle* h, int step_parity, bool write_all);
+hipError_t launch_fused_step_mom(bgk_handle* h, int step_parity, bool write_all);
+hipError_t launch_initialization_mom(bgk_handle* h);
 
 // ---- plane conversions (bgk_fused32.hip); strides in elements
 hipError_t launch_dev_to_f64(bgk_handle* h, const float* src, int sstride, double* dst,

@@ -47,8 +47,16 @@ enum {
 /* storage precision of the populations (bgk_params.precision) */
 enum {
   BGK_FP64 = 0,     /* f, g stored and computed in FP64                        */
-  BGK_FP32_DEV = 1  /* f - w*rho0 and g - w*T_crit (T_crit = 273) stored in
+  BGK_FP32_DEV = 1, /* f - w*rho0 and g - w*T_crit (T_crit = 273) stored in
                        FP32; phase field, enthalpy update and moments in FP64  */
+  BGK_FP32_MOM = 2  /* moment storage: per node rho and T - T_crit in FP64,
+                       u, v and the non-equilibrium second moments of f and g
+                       (times 1 - omega) in FP32, 48 bytes instead of 144;
+                       populations are rebuilt from them at each pull.  Clean
+                       rounding of the collision (STEP_7's FP32 temporaries
+                       ft, gt are not representable); no-slip cavity only;
+                       populations are not addressable (download/upload of
+                       f, g, fp, gp, rho2..T2 and all uploads: unsupported) */
 };
 
 /* rounding policy of the FP64 arithmetic (bgk_params.rounding) */
@@ -107,7 +115,7 @@ typedef struct bgk_params {
   int32_t east_bc;       /* id(19) 0 periodic,3 pressure,4 no-slip          */
   int32_t north_bc;      /* id(20) 0 periodic,1 velocity,4 no-slip,5 symmetric,6 open */
   int32_t south_bc;      /* id(21) 0 periodic,4 no-slip,6 open              */
-  int32_t precision;     /* BGK_FP64 or BGK_FP32_DEV                        */
+  int32_t precision;     /* BGK_FP64, BGK_FP32_DEV or BGK_FP32_MOM          */
   int32_t rounding;      /* BGK_ROUND_STEP7 or BGK_ROUND_CLEAN              */
   int32_t device;        /* HIP device ordinal                              */
   int32_t fp32_collision;/* FP32 handles: 1 = collision arithmetic in FP32,

@@ -1,0 +1,123 @@
+"""FP32 moment storage (bgk_params.precision = BGK_FP32_MOM) against the FP64
+oracle.
+
+The moment kernel rebuilds every pulled population from the upstream node's
+(rho, T) in FP64 and (u, v, Pf, Pg) in FP32, so it agrees with the clean
+FP64 policy up to the FP32 rounding of u, v and the non-equilibrium moments.
+Error measure: relative max-norm max|a - b| / max|b| per field, with the
+liquid fraction (1+phi)/2 in place of phi; on the small grids u and v are
+both scaled by the largest velocity component (in a 1 x 1 cavity u is a
+rounding residue ~1e-9 beside v ~1e-5).  Tolerances: 1e-6 on the whole
+array (halo included) after 60 steps on small and ragged grids, where FP32
+storage errors are ~1e-8; 1e-5 after 4000 steps on 64 x 64 against both
+rounding policies (the bound the FP32 population storage is held to).
+"""
+import numpy as np
+import pytest
+
+from bgk_pcm_amd.lattice import BgkError, Lattice
+from bgk_pcm_amd.params import FP32_MOM, ROUND_CLEAN, Params
+from oracle_py import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "u", "v", "T", "phi")
+
+
+def rel_errors(lat, o, interior=True, speed_scale=False):
+    out = {}
+    vmax = max(np.abs(o.field("u")).max(), np.abs(o.field("v")).max())
+    for f in FIELDS:
+        a = lat.download(f)
+        b = o.field(f)
+        if interior:
+            a, b = a[1:-1, 1:-1], b[1:-1, 1:-1]
+        if f == "phi":
+            a, b = (1.0 + a) / 2.0, (1.0 + b) / 2.0
+        scale = vmax if (speed_scale and f in ("u", "v")) else np.abs(b).max()
+        out[f] = float(np.abs(a - b).max() / scale) if scale > 0 else float(np.abs(a).max())
+    return out
+
+
+@pytest.fixture(scope="module")
+def oracles():
+    base = Params(nx=64, ny=64, length=64.0)
+    res = {}
+    for rounding in (0, 1):
+        o = Oracle(base.replace(rounding=rounding))
+        o.step(4000)
+        res[rounding] = o
+    return base, res
+
+
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_mom_storage_error_bound(oracles, rounding):
+    base, res = oracles
+    print(f"moment storage, 64x64, 4000 steps vs FP64 rounding={rounding}", flush=True)
+    with Lattice(base.replace(precision=FP32_MOM, rounding=rounding)) as lat:
+        lat.step(4000)
+        err = rel_errors(lat, res[rounding])
+    print("relative max-norm errors:", err)
+    assert all(np.isfinite(v) for v in err.values())
+    assert max(err.values()) <= 1e-5, err
+
+
+SHAPES = [(1, 1), (2, 3), (3, 2), (37, 29), (130, 5), (129, 17), (300, 9), (530, 6)]
+
+
+@pytest.mark.parametrize("nx,ny", SHAPES)
+@pytest.mark.parametrize("rows,variant", [(0, 0), (1, 1), (2, 2), (7, 3), (3, 4), (5, 5),
+                                          (2, 6), (4, 7), (1, 8)])
+def test_mom_shapes_and_strips(monkeypatch, nx, ny, rows, variant):
+    """Every grid shape, strip height (BGK_MOM_ROWS) and tiling variant
+    (BGK_MOM_VARIANT: workgroup widths 64..512, 3- and 4-row rings; 0 =
+    defaults) gives the oracle's fields, halo included, to FP32-storage
+    precision; the run is split over two bgk_step calls."""
+    monkeypatch.setenv("BGK_MOM_ROWS", str(rows))
+    monkeypatch.setenv("BGK_MOM_VARIANT", str(variant))
+    p = Params(nx=nx, ny=ny, length=float(max(nx, ny)), grav=1e-3, rounding=ROUND_CLEAN)
+    print(f"moment storage {nx}x{ny} rows={rows} variant={variant}", flush=True)
+    o = Oracle(p)
+    o.step(60)
+    with Lattice(p.replace(precision=FP32_MOM)) as lat:
+        lat.step(25)
+        lat.step(35)
+        err = rel_errors(lat, o, interior=False, speed_scale=True)
+    print("relative max-norm errors:", err)
+    assert max(err.values()) <= 1e-6, err
+
+
+def test_mom_initial_state_is_the_oracles():
+    p = Params(nx=30, ny=20, re=4.0, length=20.0)
+    o = Oracle(p)
+    with Lattice(p.replace(precision=FP32_MOM)) as lat:
+        for f in FIELDS:
+            assert np.array_equal(lat.download(f), o.field(f)), f
+
+
+def test_mom_reductions_follow_the_fields():
+    p = Params(nx=48, ny=40, length=48.0, rounding=ROUND_CLEAN)
+    o = Oracle(p)
+    o.step(300)
+    with Lattice(p.replace(precision=FP32_MOM)) as lat:
+        lat.step(300)
+        fc, fco = lat.field_check(), o.field_check()
+        np.testing.assert_allclose(fc, fco, rtol=1e-9)
+        np.testing.assert_allclose(lat.diag_sums(), o.diag_sums(), rtol=1e-6)
+
+
+def test_mom_handle_rejects_population_access():
+    with Lattice(Params(nx=8, ny=8, precision=FP32_MOM)) as lat:
+        for name in ("f3", "g0", "fp1", "rho2"):
+            with pytest.raises(BgkError) as e:
+                lat.download(name)
+            assert e.value.code == -4, name
+        with pytest.raises(BgkError) as e:
+            lat.upload("rho", np.ones((10, 10)))
+        assert e.value.code == -4
+        with pytest.raises(BgkError) as e:
+            lat.streaming()
+        assert e.value.code == -4
+    with pytest.raises(BgkError) as e:
+        Lattice(Params(nx=8, ny=8, precision=FP32_MOM, north_bc=0, south_bc=0))
+    assert e.value.code == -4
