@@ -2,26 +2,31 @@
 
 Workload: the melting cavity of the reference benchmark input
 UTIL/input.dat.0 (no-slip walls, tau_f=1, tau_g=0.51, Stefan 0.1) on a
-4096 x 4096 lattice with grav set for Ra = 1e7, advanced by the fused step with FP32
-population storage (f - w*rho0, g - w*T_crit) and FP64 arithmetic.  One "step"
-is one time step of the whole lattice (BC_f ... moments of the reference
-time loop).  Inputs are resident in HBM before the timed region.
+4096 x 4096 lattice with grav set for Ra = 1e7, advanced by the fused step
+with FP32 moment storage (per node rho and T - 273 in FP64; u, v and the
+thermal non-equilibrium moments in FP32; FP64 arithmetic).  One "step" is
+one time step of the whole lattice (BC_f ... moments of the reference time
+loop).  Inputs are resident in HBM before the timed region.
 
 Printed JSON line (one):
-  value        MLUPS of the FP32-storage step = Nx*Ny*steps/t/1e6
-  roofline     algorithmic HBM bytes per launch (160 B per lattice update:
-               18 FP32 populations read + written, phi FP64 read + written)
-               over the fused kernel's mean launch time, measured with HIP
-               events on the stream the kernel runs on; vs 8 TB/s
-  fp64         the same for the FP64 fused kernel (304 B per update)
-  fp32_error   relative max-norm error (per field, liquid fraction for phi)
-               after --error-steps of the FP32-storage run against the FP64
-               run with clean FP64 temporaries and against the STEP_7-rounding
-               FP64 run, and of the STEP_7 run against the clean one
-  cpu_baseline the reference's own STEP_OPENMP time step (compiled from its
-               sources into oracle/_ref) timed on this host's cores on a
-               bounded sample of the same workload; the CPU oracle (a port)
-               if that build is absent
+  value          MLUPS of the moment-storage step = Nx*Ny*steps/t/1e6
+  roofline       algorithmic HBM bytes per launch (88 B per lattice update
+                 with tau_f = 1: 36 B of moments read + written, phi FP64
+                 read + written; 112 B for tau_f != 1) over the fused
+                 kernel's mean launch time, measured with HIP events on the
+                 stream the kernel runs on; vs 8 TB/s
+  fp32_population the fused step with FP32 population storage (f - w*rho0,
+                 g - w*T_crit; 160 B per update), same measurements
+  fp64           the FP64 fused kernel (304 B per update)
+  fp32_error     relative max-norm error (per field, liquid fraction for
+                 phi) after --error-steps (default 20000, the config's run
+                 length) of both FP32 storages against the FP64 run with
+                 clean FP64 temporaries and against the STEP_7-rounding FP64
+                 run, and of the STEP_7 run against the clean one
+  cpu_baseline   the reference's own STEP_OPENMP time step (compiled from its
+                 sources into oracle/_ref) timed on this host's cores on a
+                 bounded sample of the same workload; the CPU oracle (a port)
+                 if that build is absent
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
                        [--error-steps E] [--cpu-steps S] [--no-cpu-baseline]
@@ -44,7 +49,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS (D2Q9 f+g) on 1 MI355X: FP32-store vs FP64 at <=1e-5 error; % HBM peak"
-BYTES_PER_LUP = {"fp32": 18 * 4 * 2 + 8 * 2, "fp64": 18 * 8 * 2 + 8 * 2}
+# algorithmic HBM bytes per lattice update: state read + written, phi
+# (FP64) read + written
+BYTES_PER_LUP = {"mom_eq": 36 * 2 + 8 * 2, "mom": 48 * 2 + 8 * 2,
+                 "fp32": 18 * 4 * 2 + 8 * 2, "fp64": 18 * 8 * 2 + 8 * 2}
 FIELDS = ("rho", "u", "v", "T", "phi")
 
 
@@ -54,12 +62,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--error-steps", type=int, default=1000)
+    ap.add_argument("--error-steps", type=int, default=20000)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--fp32-collision", action="store_true",
-                    help="FP32 collision arithmetic instead of FP64")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="after the timed region, run a device-to-device copy of this size "
                          "(reference kernel for HBM counter scaling in profiles)")
@@ -122,24 +128,27 @@ def rel_errors(a, b):
     return out
 
 
-def fp32_error(p64, p32, steps: int):
-    """Errors after `steps` steps from the same initial state: the
-    FP32-storage run against the FP64 run with clean FP64 temporaries and
-    against the STEP_7-rounding FP64 run, and the STEP_7 run against the
-    clean one (the reference's own FP32-temporary noise floor)."""
+def fp32_error(p64, storages: dict, steps: int):
+    """Errors after `steps` steps from the same initial state: each FP32
+    storage run against the FP64 run with clean FP64 temporaries and against
+    the STEP_7-rounding FP64 run, and the STEP_7 run against the clean one
+    (the reference's own FP32-temporary noise floor)."""
     from bgk_pcm_amd.lattice import Lattice
-    from bgk_pcm_amd.params import FP32_DEV, FP64, ROUND_CLEAN, ROUND_STEP7
-    assert p64.precision == FP64 and p32.precision == FP32_DEV
+    from bgk_pcm_amd.params import FP64, ROUND_CLEAN, ROUND_STEP7
+    assert p64.precision == FP64
     runs = {}
-    for name, p in (("fp64_step7", p64.replace(rounding=ROUND_STEP7)),
-                    ("fp64_clean", p64.replace(rounding=ROUND_CLEAN)), ("fp32", p32)):
+    todo = [("fp64_step7", p64.replace(rounding=ROUND_STEP7)),
+            ("fp64_clean", p64.replace(rounding=ROUND_CLEAN))] + list(storages.items())
+    for name, p in todo:
         with Lattice(p) as lat:
             lat.step(steps)
             runs[name] = {f: lat.download(f) for f in FIELDS}
-    return {"steps": steps,
-            "fp32_vs_fp64_clean": rel_errors(runs["fp32"], runs["fp64_clean"]),
-            "fp32_vs_fp64_step7": rel_errors(runs["fp32"], runs["fp64_step7"]),
-            "fp64_step7_vs_fp64_clean": rel_errors(runs["fp64_step7"], runs["fp64_clean"])}
+    out = {"steps": steps}
+    for name in storages:
+        out[f"{name}_vs_fp64_clean"] = rel_errors(runs[name], runs["fp64_clean"])
+        out[f"{name}_vs_fp64_step7"] = rel_errors(runs[name], runs["fp64_step7"])
+    out["fp64_step7_vs_fp64_clean"] = rel_errors(runs["fp64_step7"], runs["fp64_clean"])
+    return out
 
 
 def cpu_baseline(p, steps: int):
@@ -182,6 +191,21 @@ def traffic_from_profile(size: int, kind: str):
         return None
 
 
+def measure(p, stream, steps, warmup, kind, kernel):
+    """MLUPS from the wall clock and roofline figures of the fused kernel
+    from HIP events on its stream."""
+    n = p.nx
+    kms, wms, _ = gpu_run(p, stream, steps, warmup)
+    lups = p.nx * p.ny
+    nbytes = BYTES_PER_LUP[kind] * lups
+    ach = nbytes / (kms * 1e-3) / 1e9
+    return {"mlups": round(lups / (wms * 1e-3) / 1e6, 1), "ms_per_step": round(wms, 5),
+            "kernel": kernel, "kernel_ms": round(kms, 5), "achieved_gbs": round(ach, 1),
+            "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_lup": BYTES_PER_LUP[kind],
+            "algorithmic_bytes_per_launch": nbytes,
+            "traffic": traffic_from_profile(n, kind)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -189,63 +213,65 @@ def main():
               "(the multi-GPU slab decomposition is design-only)", file=sys.stderr)
         sys.exit(2)
     import torch
-    from bgk_pcm_amd.params import FP32_DEV
+    from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, ROUND_CLEAN
 
     torch.cuda.set_device(0)
     stream = torch.cuda.Stream()
     n = args.size
     p = cavity(n)
-    p32 = p.replace(precision=FP32_DEV, fp32_collision=1 if args.fp32_collision else 0)
+    # the moment and FP32-population kernels follow the clean rounding of the
+    # collision; beta (forces.f90:13) is then FP64 as well
+    pm = p.replace(precision=FP32_MOM, rounding=ROUND_CLEAN)
+    pp = p.replace(precision=FP32_DEV, rounding=ROUND_CLEAN)
+    mom_kind = "mom_eq" if p.tau_f == 1.0 else "mom"
 
-    k32, w32, _ = gpu_run(p32, stream, args.steps, args.warmup)
-    lups = n * n
-    mlups32 = lups * args.steps / (w32 * 1e-3 * args.steps) / 1e6
-    bytes32 = BYTES_PER_LUP["fp32"] * lups
-    ach32 = bytes32 / (k32 * 1e-3) / 1e9
+    mm = measure(pm, stream, args.steps, args.warmup, mom_kind, "k_fused_mom")
     res = {
         "metric": METRIC,
-        "value": round(mlups32, 1),
+        "value": mm["mlups"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(w32, 5),
+        "ms_per_step": mm["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.fp32_collision else "f64",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"melting cavity (UTIL/input.dat.0 physics), {n}x{n}, fused time step, "
-                        "FP32 population storage",
-            "nx": n, "ny": n, "storage": "f32 deviations from rest equilibrium",
-            "arithmetic": "f32 collision, f64 elsewhere" if args.fp32_collision else "f64",
+            "workload": f"melting cavity (UTIL/input.dat.0 physics, Ra 1e7), {n}x{n}, "
+                        "fused time step, FP32 moment storage",
+            "nx": n, "ny": n,
+            "storage": "per node rho, T-273 FP64; u, v, (1-omega_g) Pi_g^neq FP32"
+                       + ("" if mom_kind == "mom_eq" else ", (1-omega_f) Pi_f^neq FP32"),
+            "arithmetic": "f64",
             "boundaries": "no-slip (type 4) on all walls",
             "tau_f": p.tau_f, "tau_g": p.tau_g, "grav": p.grav, "rayleigh": RAYLEIGH,
             "stefan_number": p.stefan_number,
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": round(ach32, 1),
+            "achieved": mm["achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(ach32 / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profile(n, "fp32"),
-            "kernel": "k_fused32",
-            "kernel_ms": round(k32, 5),
-            "algorithmic_bytes_per_launch": bytes32,
+            "frac": mm["frac"],
+            "traffic": mm["traffic"],
+            "kernel": mm["kernel"],
+            "kernel_ms": mm["kernel_ms"],
+            "bytes_per_lup": mm["bytes_per_lup"],
+            "algorithmic_bytes_per_launch": mm["algorithmic_bytes_per_launch"],
         },
     }
+    res["fp32_population"] = measure(pp, stream, args.steps, args.warmup, "fp32", "k_fused32")
     if not args.no_fp64:
-        k64, w64, _ = gpu_run(p, stream, max(args.steps // 2, 1), args.warmup)
-        ach64 = BYTES_PER_LUP["fp64"] * lups / (k64 * 1e-3) / 1e9
-        res["fp64"] = {"mlups": round(lups / (w64 * 1e-3) / 1e6, 1), "ms_per_step": round(w64, 5),
-                       "kernel_ms": round(k64, 5), "achieved_gbs": round(ach64, 1),
-                       "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                       "traffic": traffic_from_profile(n, "fp64")}
-        res["fp32_over_fp64"] = round(res["value"] / res["fp64"]["mlups"], 3)
+        res["fp64"] = measure(p, stream, max(args.steps // 2, 1), args.warmup, "fp64",
+                              "k_fused64")
+        res["mom_over_fp64"] = round(res["value"] / res["fp64"]["mlups"], 3)
+        res["fp32_population_over_fp64"] = round(
+            res["fp32_population"]["mlups"] / res["fp64"]["mlups"], 3)
     if args.error_steps > 0:
-        res["fp32_error"] = fp32_error(p, p32, args.error_steps)
+        res["fp32_error"] = fp32_error(p, {"mom": pm, "fp32_population": pp}, args.error_steps)
     if args.calibrate_copy > 0:
         # copies of known size for counter scaling: the runtime's device copy,
         # then 8-byte plain and non-temporal copies (libbgk_calib.so), the

@@ -91,7 +91,8 @@ size_t carve(bgk_handle* h, char* base) {
   const bool fp32 = prec == BGK_FP32_DEV, mom = prec == BGK_FP32_MOM;
   for (int b = 0; b < 2; ++b) {
     h->ma[b] = nullptr;
-    h->mb[b] = h->mc[b] = nullptr;
+    h->mb[b] = nullptr;
+    h->mc[b] = nullptr;
     for (int k = 0; k < 9; ++k) {
       h->f[b][k] = h->g[b][k] = nullptr;
       h->f32[b][k] = h->g32[b][k] = nullptr;
@@ -99,7 +100,8 @@ size_t carve(bgk_handle* h, char* base) {
     if (mom) {
       h->ma[b] = cv.plane<double2>(n);
       h->mb[b] = cv.plane<float4>(n);
-      h->mc[b] = cv.plane<float4>(n);
+      h->mc[b] = h->mom_eqf ? static_cast<void*>(cv.plane<float>(n))
+                            : static_cast<void*>(cv.plane<float4>(n));
       continue;
     }
     for (int k = 0; k < 9; ++k) {
@@ -293,6 +295,9 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
   if (p->device < 0 || p->device >= ndev)
     return fail(BGK_E_NODEVICE, "device %d out of range (%d visible)", p->device, ndev);
   HIP_TRY(hipSetDevice(p->device), "hipSetDevice");
+  int n_cu = 0;
+  HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device),
+          "hipDeviceGetAttribute");
 
   bgk_handle* h = new (std::nothrow) bgk_handle();
   if (!h) return fail(BGK_E_NOMEM, "host allocation");
@@ -300,6 +305,7 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
   h->nx = p->nx;
   h->ny = p->ny;
   h->device = p->device;
+  h->n_cu = n_cu;
   h->pitch = row_pitch(p->nx);
   h->plane_elems = (int64_t)(p->ny + 2) * h->pitch;
   h->small_elems = std::max<int64_t>(4096, 3 * (int64_t)(p->nx + p->ny + 4));
@@ -307,12 +313,13 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
   h->fused_ok = fused_ok;
   h->cur = 0;
   h->pc = 0;
-  // tiling of the moment kernel: BGK_MOM_ROWS (rows per strip, default 32)
-  // and BGK_MOM_VARIANT (1..8: workgroup width 64/128/256/512 with a 4- or
+  // tiling of the moment kernel: BGK_MOM_ROWS (rows per strip; default
+  // about 64, or 32 for tau_f != 1, fitted to the device) and BGK_MOM_VARIANT (1..8: workgroup width 64/128/256/512 with a 4- or
   // 3-row ring; 0: chosen from Nx) override the defaults; tuning knobs,
   // every value is correct
   h->strip_rows = 0;
   h->mom_variant = 0;
+  h->mom_eqf = (1.0 / p->tau_f) == 1.0;
   if (const char* env = std::getenv("BGK_MOM_ROWS")) h->strip_rows = std::max(0, std::atoi(env));
   if (const char* env = std::getenv("BGK_MOM_VARIANT")) h->mom_variant = std::max(0, std::atoi(env));
   set_constants(h);

@@ -28,6 +28,8 @@
 // Arithmetic is FP64 throughout.  STEP_7's FP32 temporaries (ft, gt) cannot
 // be represented by moments, so this path follows the clean rounding policy;
 // beta follows bgk_params.rounding as everywhere.
+#include <algorithm>
+
 #include "bgk_internal.h"
 
 namespace bgk {
@@ -62,20 +64,33 @@ struct Macro {
   double *rho, *u, *v, *T;
 };
 
+// Element of the third plane.  With omega_f = 1 (tau_f = 1, the reference
+// inputs) the regularised flow collision returns the equilibrium, Pf = 0,
+// and a node needs 36 bytes: B = (u, v, Pg_xx, Pg_yy), C = Pg_xy.
+template <bool EQF>
+struct CPlane {
+  using T = float4;  // Pf_xy, Pg_xx, Pg_yy, Pg_xy
+};
+template <>
+struct CPlane<true> {
+  using T = float;  // Pg_xy
+};
+
 // Moments of one node as held in the ring.
+template <bool EQF>
 struct NodeM {
   double2 a;  // rho, T - T_crit
-  float4 b;   // u, v, Pf_xx, Pf_yy
-  float4 c;   // Pf_xy, Pg_xx, Pg_yy, Pg_xy
+  float4 b;   // u, v, Pf_xx, Pf_yy  (EQF: u, v, Pg_xx, Pg_yy)
+  typename CPlane<EQF>::T c;
   double phi;
 };
 
 // LDS ring of NS rows of W + 2 nodes (W columns and the two halo columns).
-template <int W, int NS>
+template <int W, int NS, bool EQF>
 struct Ring {
   double2 a[NS][W + 2];
   float4 b[NS][W + 2];
-  float4 c[NS][W + 2];
+  typename CPlane<EQF>::T c[NS][W + 2];
   double phi[NS][W + 2];
 };
 
@@ -120,12 +135,22 @@ __device__ __forceinline__ void rebuild(int k, const double2 a, const float4 b, 
   *fk = kW[k] * a.x * (1.0 + p) + coef_of(k) * hf;
   *gdk = kW[k] * (a.y + (kTcrit + a.y) * p) + coef_of(k) * hg;
 }
+__device__ __forceinline__ void rebuild(int k, const double2 a, const float4 b, const float cc,
+                                        double* fk, double* gdk) {
+  const double p = eq_poly(k, (double)b.x, (double)b.y);
+  const double hg = (double)h_k32(k, b.z, b.w, cc);
+  *fk = kW[k] * a.x * (1.0 + p);
+  *gdk = kW[k] * (a.y + (kTcrit + a.y) * p) + coef_of(k) * hg;
+}
 
 __device__ __forceinline__ double rebuild_f(int k, const double2 a, const float4 b,
                                             const float4 cc) {
   const double p = eq_poly(k, (double)b.x, (double)b.y);
   const double hf = (double)h_k32(k, b.z, b.w, cc.x);
   return kW[k] * a.x * (1.0 + p) + coef_of(k) * hf;
+}
+__device__ __forceinline__ double rebuild_f(int k, const double2 a, const float4 b, float) {
+  return kW[k] * a.x * (1.0 + eq_poly(k, (double)b.x, (double)b.y));
 }
 
 // Equilibrium deviation of a boundary ghost, g_k - w_k*T_crit.
@@ -136,13 +161,25 @@ __device__ __forceinline__ double ghost_gdev(int k, double T, double u, double v
 struct MomIn {
   const double2* a;
   const float4* b;
-  const float4* c;
+  const void* c;  // float4 or float (CPlane)
 };
 struct MomOut {
   double2* a;
   float4* b;
-  float4* c;
+  void* c;
 };
+
+template <bool EQF>
+__device__ __forceinline__ typename CPlane<EQF>::T c_at(const void* c, int64_t q) {
+  return static_cast<const typename CPlane<EQF>::T*>(c)[q];
+}
+template <bool EQF>
+__device__ __forceinline__ typename CPlane<EQF>::T c_stream(const void* c, int64_t q) {
+  if constexpr (EQF)
+    return __builtin_nontemporal_load(static_cast<const float*>(c) + q);
+  else
+    return stream_load4(static_cast<const float4*>(c) + q);
+}
 
 // Row strips: workgroup b runs on XCD b % 8; all column segments of a strip
 // go to one XCD (its halo columns are L2 hits) and the eight XCDs work on
@@ -154,25 +191,26 @@ struct StripGrid {
   int nmain;   // workgroups of the strips (padded to a multiple of 8 strips)
 };
 
-__device__ __forceinline__ NodeM load_node(const MomIn& in, const double* __restrict__ pa,
-                                           const Geom& gm, int ic, int jr) {
-  NodeM n;
+template <bool EQF>
+__device__ __forceinline__ NodeM<EQF> load_node(const MomIn& in, const double* __restrict__ pa,
+                                                const Geom& gm, int ic, int jr) {
+  NodeM<EQF> n;
   n.a = make_double2(0.0, 0.0);
   n.b = make_float4(0.f, 0.f, 0.f, 0.f);
-  n.c = n.b;
+  n.c = typename CPlane<EQF>::T{};
   n.phi = 0.0;
   const int64_t q = (int64_t)jr * gm.pitch + ic;
   if (ic >= 1 && ic <= gm.nx) {
     n.a = stream_load(in.a + q);
     n.b = stream_load4(in.b + q);
-    n.c = stream_load4(in.c + q);
+    n.c = c_stream<EQF>(in.c, q);
   }
   if (ic >= 0 && ic <= gm.nx + 1) n.phi = pa[q];
   return n;
 }
 
-template <int W, int NS>
-__device__ __forceinline__ void put(Ring<W, NS>& L, int r, int s, const NodeM& n) {
+template <int W, int NS, bool EQF>
+__device__ __forceinline__ void put(Ring<W, NS, EQF>& L, int r, int s, const NodeM<EQF>& n) {
   L.a[r][s] = n.a;
   L.b[r][s] = n.b;
   L.c[r][s] = n.c;
@@ -252,6 +290,7 @@ struct NodeOut {
   double rho, Tav, uav, vav;
 };
 
+template <bool EQF>
 __device__ __forceinline__ NodeOut update_node(const Consts& c, const PullSums& S,
                                                const double tap[9], const MomOut& out,
                                                double* __restrict__ pb, int64_t q) {
@@ -283,17 +322,24 @@ __device__ __forceinline__ NodeOut update_node(const Consts& c, const PullSums& 
   // non-equilibrium second moments of the pulled populations about the
   // equilibria at (rho, u, v) and (T, u, v); for the second-order D2Q9
   // equilibrium sum_k H_k feq_k = rho u u.
-  const double omf1 = 1.0 - c.omega_f, omg1 = 1.0 - c.omega_g;
-  const double pfxx = omf1 * (S.fxx - kCs2 * rho - rho * u * u);
-  const double pfyy = omf1 * (S.fyy - kCs2 * rho - rho * v * v);
-  const double pfxy = omf1 * (S.fxy - rho * u * v);
+  const double omg1 = 1.0 - c.omega_g;
   const double pgxx = omg1 * (S.gxx - kCs2 * dT2 - T * u * u);
   const double pgyy = omg1 * (S.gyy - kCs2 * dT2 - T * v * v);
   const double pgxy = omg1 * (S.gxy - T * u * v);
   const float us = (float)u, vs = (float)v;
   stream_store(out.a + q, rho, dT);
-  stream_store4(out.b + q, us, vs, (float)pfxx, (float)pfyy);
-  stream_store4(out.c + q, (float)pfxy, (float)pgxx, (float)pgyy, (float)pgxy);
+  if constexpr (EQF) {
+    stream_store4(out.b + q, us, vs, (float)pgxx, (float)pgyy);
+    __builtin_nontemporal_store((float)pgxy, static_cast<float*>(out.c) + q);
+  } else {
+    const double omf1 = 1.0 - c.omega_f;
+    const double pfxx = omf1 * (S.fxx - kCs2 * rho - rho * u * u);
+    const double pfyy = omf1 * (S.fyy - kCs2 * rho - rho * v * v);
+    const double pfxy = omf1 * (S.fxy - rho * u * v);
+    stream_store4(out.b + q, us, vs, (float)pfxx, (float)pfyy);
+    stream_store4(static_cast<float4*>(out.c) + q, (float)pfxy, (float)pgxx, (float)pgyy,
+                  (float)pgxy);
+  }
   stream_store(&pb[q], phi);
 
   // ---- moments of the stored state (STEP_7/moments.f90:11-25): sum_k f_k =
@@ -311,13 +357,13 @@ __device__ __forceinline__ NodeOut update_node(const Consts& c, const PullSums& 
 // ring.  W columns per workgroup, NS rows in the ring (4: one barrier per
 // row; 3: two, for less LDS).  Workgroups from sg.nmain on serve the
 // boundary-adjacent nodes (mom_edge_nodes).
-template <int W, int NS>
+template <int W, int NS, bool EQF>
 __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
                                              const double* __restrict__ pa,
                                              double* __restrict__ pb, const Macro& m,
                                              const Consts& c, const Geom& gm, const StripGrid& sg,
                                              int write_all) {
-  __shared__ Ring<W, NS> L;
+  __shared__ Ring<W, NS, EQF> L;
   const int nx = gm.nx, ny = gm.ny;
   const int64_t P = gm.pitch;
   const int blk = (int)blockIdx.x;
@@ -338,17 +384,17 @@ __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
   const bool col_in = (i > 1 && i < nx);
 
   for (int jj = j0 - 1; jj <= j0 + 1; ++jj) {
-    put(L, slot(jj), t + 1, load_node(in, pa, gm, i, clampj(jj)));
-    if (t < 2) put(L, slot(jj), hslot, load_node(in, pa, gm, hcol, clampj(jj)));
+    put(L, slot(jj), t + 1, load_node<EQF>(in, pa, gm, i, clampj(jj)));
+    if (t < 2) put(L, slot(jj), hslot, load_node<EQF>(in, pa, gm, hcol, clampj(jj)));
   }
   __syncthreads();
 
   for (int j = j0; j <= j1; ++j) {
     const bool pre = (j + 2 <= j1 + 1);
-    NodeM nn, nh;
+    NodeM<EQF> nn, nh;
     if (pre) {
-      nn = load_node(in, pa, gm, i, clampj(j + 2));
-      if (t < 2) nh = load_node(in, pa, gm, hcol, clampj(j + 2));
+      nn = load_node<EQF>(in, pa, gm, i, clampj(j + 2));
+      if (t < 2) nh = load_node<EQF>(in, pa, gm, hcol, clampj(j + 2));
     }
     if (col_in && j > 1 && j < ny) {
       const int rS = slot(j - 1), rC = slot(j), rN = slot(j + 1);
@@ -368,7 +414,7 @@ __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
       const double tap[9] = {L.phi[rC][s],     L.phi[rC][s + 1], L.phi[rN][s],
                              L.phi[rC][s - 1], L.phi[rS][s],     L.phi[rN][s + 1],
                              L.phi[rN][s - 1], L.phi[rS][s - 1], L.phi[rS][s + 1]};
-      const NodeOut o = update_node(c, S, tap, out, pb, q);
+      const NodeOut o = update_node<EQF>(c, S, tap, out, pb, q);
       if (write_all) {
         m.rho[q] = o.rho;
         m.T[q] = o.Tav;
@@ -408,6 +454,7 @@ __device__ __forceinline__ bool edge_node(int e, int nx, int ny, int* i, int* j)
 // frozen corner ghosts, W/E thermal ghosts at Tin and at the previous
 // step's averaged T(Nx, j), N/S ghost rows of g and phi equal to rows Ny / 1;
 // then the halo bookkeeping of the macroscopic fields.
+template <bool EQF>
 __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& out,
                                                const double* __restrict__ pa,
                                                double* __restrict__ pb,
@@ -420,7 +467,8 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
   if (!edge_node(e, nx, ny, &i, &j)) return;
   const int64_t q = (int64_t)j * P + i;
   const double2 oa = in.a[q];
-  const float4 ob = in.b[q], oc = in.c[q];
+  const float4 ob = in.b[q];
+  const typename CPlane<EQF>::T oc = c_at<EQF>(in.c, q);
   double f[9], gd[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
@@ -430,7 +478,7 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
       // interior source column: rebuild from the node (si, sjc); for a ghost
       // row this is the N/S copy of g, and f is replaced below
       const int64_t h = (int64_t)sjc * P + si;
-      rebuild(k, in.a[h], in.b[h], in.c[h], &f[k], &gd[k]);
+      rebuild(k, in.a[h], in.b[h], c_at<EQF>(in.c, h), &f[k], &gd[k]);
     } else {
       const int64_t h = (int64_t)sjc * P + si;
       gd[k] = ghost_gdev(k, si == 0 ? c.tin : te_in[sjc], m.u[h], m.v[h]);
@@ -453,7 +501,7 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
   pull_zero(S);
 #pragma unroll
   for (int k = 0; k < 9; ++k) pull_add(S, k, f[k], gd[k]);
-  const NodeOut o = update_node(c, S, tap, out, pb, q);
+  const NodeOut o = update_node<EQF>(c, S, tap, out, pb, q);
 
   // ---- halo bookkeeping: BC_f N/S rows (rho copied, u = v = 0), BC_g
   // T and phi ghosts, the east column's averaged T for the next step
@@ -507,21 +555,22 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
   m.v[q] = o.vav;
 }
 
-template <int W, int NS>
+template <int W, int NS, bool EQF>
 __global__ __launch_bounds__(W) void k_fused_mom(
     MomIn in, MomOut out, const double* __restrict__ pa, double* __restrict__ pb,
     const double* __restrict__ te_in, double* __restrict__ te_out, Macro m, Consts c, Geom gm,
     StripGrid sg, int write_all) {
   if ((int)blockIdx.x >= sg.nmain) {
     const int e = ((int)blockIdx.x - sg.nmain) * W + (int)threadIdx.x;
-    mom_edge_nodes(in, out, pa, pb, te_in, te_out, m, c, gm, e);
+    mom_edge_nodes<EQF>(in, out, pa, pb, te_in, te_out, m, c, gm, e);
     return;
   }
-  mom_interior<W, NS>(in, out, pa, pb, m, c, gm, sg, write_all);
+  mom_interior<W, NS, EQF>(in, out, pa, pb, m, c, gm, sg, write_all);
 }
 
 // STEP_7/initialization.f90:15-152 for a moment handle: the initial
 // populations are equilibria, i.e. moments with zero non-equilibrium part.
+template <bool EQF>
 __global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* phip, Consts c,
                                      Geom gm, int interf, int west_bc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -531,7 +580,7 @@ __global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* ph
   const NodeInit s0 = initial_node(c, gm, i, j, interf, west_bc);
   st.a[q] = make_double2(s0.rho, s0.T - kTcrit);
   st.b[q] = make_float4((float)s0.u, (float)s0.v, 0.f, 0.f);
-  st.c[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  static_cast<typename CPlane<EQF>::T*>(st.c)[q] = typename CPlane<EQF>::T{};
   m.rho[q] = s0.rho;
   m.u[q] = s0.u;
   m.v[q] = s0.v;
@@ -540,26 +589,60 @@ __global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* ph
   phip[q] = s0.phi;
 }
 
-StripGrid make_strip_grid(const bgk_handle* h, int w) {
+// Row strips of about `target` rows.  The strip count is chosen, where one
+// exists within a factor two, as a multiple of 8 (the XCDs) whose workgroups
+// fill whole rounds of the device (slots = CUs x resident workgroups per
+// CU): measured at 4096^2, 64-row strips (512 workgroups) run at 57 GLUPS
+// and 96-row strips (43 strips, 1.5 rounds) at 43.
+StripGrid make_strip_grid(const bgk_handle* h, int w, int target, int slots) {
   StripGrid s;
+  const int ny = h->ny;
   s.nseg = (h->nx + w - 1) / w;
-  s.rows = h->strip_rows > 0 ? h->strip_rows : 32;
-  s.nstrip = (h->ny + s.rows - 1) / s.rows;
+  if (h->strip_rows > 0) {
+    s.rows = h->strip_rows;
+  } else {
+    const int base = std::max(1, (ny + target - 1) / target);
+    int pick = base;
+    for (int n = base; n <= 2 * base && n <= ny; ++n)
+      if (n % NXCD == 0 && slots > 0 && ((int64_t)n * s.nseg) % slots == 0) {
+        pick = n;
+        break;
+      }
+    s.rows = (ny + pick - 1) / pick;
+  }
+  s.nstrip = (ny + s.rows - 1) / s.rows;
   s.nmain = ((s.nstrip + NXCD - 1) / NXCD) * NXCD * s.nseg;
   return s;
+}
+
+template <int W, int NS, bool EQF>
+void launch_mom_t(bgk_handle* h, MomIn in, MomOut out, const double* te_in, double* te_out,
+                  bool write_all) {
+  // resident workgroups per CU of this instantiation (LDS and registers)
+  static const int per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_mom<W, NS, EQF>, W, 0) !=
+        hipSuccess)
+      n = 0;
+    return n;
+  }();
+  const StripGrid sg = make_strip_grid(h, W, EQF ? 64 : 32, per_cu * h->n_cu);
+  const int nx = h->nx, ny = h->ny;
+  const int nedge = nx * (ny > 1 ? 2 : 1) + (ny > 2 ? (ny - 2) * (nx > 1 ? 2 : 1) : 0);
+  const int64_t nwg = (int64_t)sg.nmain + (nedge + W - 1) / W;
+  Macro m{h->rho, h->u, h->v, h->T};
+  hipLaunchKernelGGL((k_fused_mom<W, NS, EQF>), dim3((unsigned)nwg), dim3(W), 0, h->stream, in,
+                     out, h->phi[h->pc], h->phi[1 - h->pc], te_in, te_out, m, h->c, h->geom, sg,
+                     write_all ? 1 : 0);
 }
 
 template <int W, int NS>
 void launch_mom(bgk_handle* h, MomIn in, MomOut out, const double* te_in, double* te_out,
                 bool write_all) {
-  const StripGrid sg = make_strip_grid(h, W);
-  const int nx = h->nx, ny = h->ny;
-  const int nedge = nx * (ny > 1 ? 2 : 1) + (ny > 2 ? (ny - 2) * (nx > 1 ? 2 : 1) : 0);
-  const int64_t nwg = (int64_t)sg.nmain + (nedge + W - 1) / W;
-  Macro m{h->rho, h->u, h->v, h->T};
-  hipLaunchKernelGGL((k_fused_mom<W, NS>), dim3((unsigned)nwg), dim3(W), 0, h->stream, in,
-                     out, h->phi[h->pc], h->phi[1 - h->pc], te_in, te_out, m, h->c, h->geom, sg,
-                     write_all ? 1 : 0);
+  if (h->mom_eqf)
+    launch_mom_t<W, NS, true>(h, in, out, te_in, te_out, write_all);
+  else
+    launch_mom_t<W, NS, false>(h, in, out, te_in, te_out, write_all);
 }
 
 }  // namespace
@@ -570,12 +653,14 @@ hipError_t launch_fused_step_mom(bgk_handle* h, int step_parity, bool write_all)
   const int a = h->cur, b = 1 - h->cur;
   MomIn in{h->ma[a], h->mb[a], h->mc[a]};
   MomOut out{h->ma[b], h->mb[b], h->mc[b]};
-  // Tiling (workgroup width W, ring rows NS).  Measured at 4096^2 on MI355X
-  // (rows per strip 32): W = 512 / NS = 3 46.3 GLUPS, 512/4 46.0, 256/4
-  // 45.4 (64-row strips), 128/4 41.2, 64/4 40.4.  Wider segments stream
-  // longer contiguous runs per row; W = 512 leaves one workgroup per CU.
+  // Tiling (workgroup width W, ring rows NS).  Measured at 4096^2 on MI355X:
+  // omega_f = 1 (36-byte nodes, 64-row strips) W = 512 / NS = 4 57.2 GLUPS,
+  // 512/3 53.1, 256/4 53.9 (32-row strips), 128/4 45.1, 64/4 45.8; general
+  // layout (48 bytes, 32-row strips) 512/3 45.9, 512/4 45.6, 256/3 42.0.
+  // Wider segments stream longer contiguous runs of each row.
   int v = h->mom_variant;
-  if (v == 0) v = h->nx >= 2048 ? 8 : h->nx >= 512 ? 5 : h->nx >= 128 ? 3 : 1;
+  if (v == 0)
+    v = h->nx >= 2048 ? (h->mom_eqf ? 7 : 8) : h->nx >= 512 ? 5 : h->nx >= 128 ? 3 : 1;
   switch (v) {
     case 1: launch_mom<64, 4>(h, in, out, te_in, te_out, write_all); break;
     case 2: launch_mom<64, 3>(h, in, out, te_in, te_out, write_all); break;
@@ -599,8 +684,12 @@ hipError_t launch_initialization_mom(bgk_handle* h) {
   const Geom& g = h->geom;
   MomOut st{h->ma[h->cur], h->mb[h->cur], h->mc[h->cur]};
   dim3 grd((unsigned)((g.nx + 2 + TX - 1) / TX), (unsigned)((g.ny + 2 + TY - 1) / TY));
-  hipLaunchKernelGGL(k_initialization_mom, grd, dim3(TX, TY), 0, h->stream, st, m, h->phi[h->pc],
-                     h->phi[1 - h->pc], h->c, g, g.nx / 40, h->prm.west_bc);
+  if (h->mom_eqf)
+    hipLaunchKernelGGL(k_initialization_mom<true>, grd, dim3(TX, TY), 0, h->stream, st, m,
+                       h->phi[h->pc], h->phi[1 - h->pc], h->c, g, g.nx / 40, h->prm.west_bc);
+  else
+    hipLaunchKernelGGL(k_initialization_mom<false>, grd, dim3(TX, TY), 0, h->stream, st, m,
+                       h->phi[h->pc], h->phi[1 - h->pc], h->c, g, g.nx / 40, h->prm.west_bc);
   return hipGetLastError();
 }
 

@@ -24,13 +24,15 @@
 // g[b][k] = f[b][k] + 1.  phi[pc] is phi and phi[1-pc] phi_prec.  FP32
 // handles hold f32/g32 = f - w*rho0, g - w*273 (T_crit) the same way.
 // Moment handles hold per node ma = (rho, T - 273) FP64, mb = (u, v, Pf_xx,
-// Pf_yy) and mc = (Pf_xy, Pg_xx, Pg_yy, Pg_xy) FP32, double-buffered by cur.
+// Pf_yy) and mc = (Pf_xy, Pg_xx, Pg_yy, Pg_xy) FP32, double-buffered by cur
+// (with omega_f = 1: mb = (u, v, Pg_xx, Pg_yy), mc = Pg_xy).
 struct bgk_handle {
   bgk_params prm;
   int nx, ny;
   int64_t pitch;        // elements per row
   int64_t plane_elems;  // nodes per plane: (ny+2)*pitch
   int device;
+  int n_cu;  // compute units of the device
   hipStream_t own_stream;
   hipStream_t stream;
 
@@ -46,7 +48,8 @@ struct bgk_handle {
   // FP32 moment state (moment handles)
   double2* ma[2];
   float4* mb[2];
-  float4* mc[2];
+  void* mc[2];    // float4 planes, or float planes when mom_eqf
+  bool mom_eqf;   // omega_f == 1: Pf vanishes, mb = (u, v, Pg_xx, Pg_yy), mc = Pg_xy
   int strip_rows;   // rows per workgroup strip of the moment kernel (0: default)
   int mom_variant;  // tiling variant of the moment kernel (0: default)
   int cur;

@@ -206,8 +206,10 @@ contains
     write(*,*) '****************************************************'
     if (prm%precision == 0) then
        write(*,*) 'Double precision used (GPU)                         '
-    else
+    else if (prm%precision == 1) then
        write(*,*) 'FP32 population storage used (GPU)                  '
+    else
+       write(*,*) 'FP32 moment storage used (GPU)                      '
     end if
     write(*,*) '****************************************************'
     write(*,*) 'Nx   = ', prm%nx

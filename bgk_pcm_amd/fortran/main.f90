@@ -15,6 +15,7 @@
 !   --stages   call the eight subroutines per step instead (BC_f, BC_g,
 !              streaming, momentsPre, phase_field, forces, collision, moments)
 !   --fp32     FP32 population storage
+!   --mom      FP32 moment storage (rho and T in FP64; clean rounding)
 !   --clean    FP64 temporaries where STEP_7 has implicit REAL(4) ones
 program bgk_thermal_gpu
   use iso_c_binding
@@ -42,6 +43,9 @@ program bgk_thermal_gpu
         per_stage = .true.
      case ('--fp32')
         p%precision = 1
+     case ('--mom')
+        p%precision = 2
+        p%rounding = 1
      case ('--clean')
         p%rounding = 1
      case default
