@@ -1,0 +1,717 @@
+// bgk_adjoint.hip -- discrete adjoint of the moment-storage time step.
+//
+// Forward model: the FP64 moment-storage step (bgk_fused_mom.hip, clean
+// rounding, no-slip cavity, extensions t_hot and k_ratio).  Loss: any
+// function of the averaged temperature T (STEP_7/moments.f90:11-25) at a
+// set of probe nodes after every step; the caller supplies dL/dT(step,
+// probe) and receives dL/d(stefan_number, k_ratio, t_hot).
+//
+// Adjoint state per node (step n): the adjoints of the ten pulled moments
+// the node update depends on (PullSums of bgk_mom.h: rho, j, the second
+// moments of f; T and the second moments of g) and of the two tap sums and
+// the centre tap of the phase field, 13 doubles.  The pull fp_k(x) =
+// f_k(x - c_k) is reversed as a gather, so no atomics: node y collects,
+// for every direction k, the adjoint of the population k that node y + c_k
+// pulled from it, and turns it into the adjoint of its own stored moments
+// through the derivative of the population rebuild.  The node update is
+// reversed by hand (core_backward below), on values recomputed with the
+// same core_forward the forward kernels run.
+//
+// Checkpointing: bgk_adj_forward keeps the state every seg_len steps;
+// bgk_adj_backward recomputes one segment at a time from its checkpoint,
+// keeping every state of the segment, then runs the adjoint kernels over
+// the segment backwards.  Memory: (nsteps/seg_len + seg_len + 1) states.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "bgk_mom.h"
+
+namespace bgk {
+namespace {
+
+using namespace mom;
+
+constexpr int TX = 64, TY = 4;
+
+// one stored state: moment planes, phi, east-wall column
+struct StatePtrs {
+  double2* a;
+  void* b;
+  void* c;
+  double* phi;
+  double* te;
+};
+
+// adjoint planes of one step (structure of arrays, node q at [q])
+struct AdjPlanes {
+  double* v[13];  // rho jx jy fxx fyy fxy dT gxx gyy gxy kp km pc
+};
+enum { L_RHO, L_JX, L_JY, L_FXX, L_FYY, L_FXY, L_DT, L_GXX, L_GYY, L_GXY, L_KP, L_KM, L_PC };
+
+struct AdjArgs {
+  // forward: state after step n (gather derivatives) and the inputs of step n
+  StatePtrs sn, sprev;
+  const double* uh;  // macroscopic u, v planes: their halo columns are the ghost velocities
+  const double* vh;
+  AdjPlanes next;    // adjoint of step n+1 (read)
+  AdjPlanes cur;     // adjoint of step n (written)
+  int has_next;      // 0 at the last step: no step n+1
+  int core;          // 0: only the gather (the initial state)
+  const int* probe_idx;
+  const double* seeds;  // dL/dT at the probes after step n (null: none)
+  double* partial;      // 3 per workgroup: dL/dSt, dL/dk_ratio, dL/dt_hot (accumulated)
+  Consts c;
+  Geom gm;
+  int interf;
+};
+
+// Adjoint of population k as pulled by node q, from that node's adjoint of
+// its pulled moments (PullSums is linear in the populations).
+__device__ __forceinline__ double lam_f(const AdjPlanes& L, int k, int64_t q) {
+  double r = L.v[L_RHO][q];
+  if (CX[k] != 0) r += (CX[k] > 0 ? L.v[L_JX][q] : -L.v[L_JX][q]) + L.v[L_FXX][q];
+  if (CY[k] != 0) r += (CY[k] > 0 ? L.v[L_JY][q] : -L.v[L_JY][q]) + L.v[L_FYY][q];
+  if (CX[k] * CY[k] != 0) r += CX[k] * CY[k] > 0 ? L.v[L_FXY][q] : -L.v[L_FXY][q];
+  return r;
+}
+__device__ __forceinline__ double lam_g(const AdjPlanes& L, int k, int64_t q) {
+  double r = L.v[L_DT][q];
+  if (CX[k] != 0) r += L.v[L_GXX][q];
+  if (CY[k] != 0) r += L.v[L_GYY][q];
+  if (CX[k] * CY[k] != 0) r += CX[k] * CY[k] > 0 ? L.v[L_GXY][q] : -L.v[L_GXY][q];
+  return r;
+}
+
+// Stored moments of a node in FP64 (EQF: Pf = 0).
+struct Mom {
+  double rho, th, u, v, pf[3], pg[3];
+};
+
+template <bool EQF>
+__device__ __forceinline__ Mom load_mom(const StatePtrs& s, int64_t q) {
+  Mom m;
+  const double2 a = s.a[q];
+  const double4 b = static_cast<const double4*>(s.b)[q];
+  m.rho = a.x;
+  m.th = a.y;
+  m.u = b.x;
+  m.v = b.y;
+  if constexpr (EQF) {
+    m.pf[0] = m.pf[1] = m.pf[2] = 0.0;
+    m.pg[0] = b.z;
+    m.pg[1] = b.w;
+    m.pg[2] = static_cast<const double*>(s.c)[q];
+  } else {
+    const double4 cc = static_cast<const double4*>(s.c)[q];
+    m.pf[0] = b.z;
+    m.pf[1] = b.w;
+    m.pf[2] = cc.x;
+    m.pg[0] = cc.y;
+    m.pg[1] = cc.z;
+    m.pg[2] = cc.w;
+  }
+  return m;
+}
+
+// d(H_k : P)/dP for P = (xx, yy, xy)
+__device__ __forceinline__ void h_coef(int k, double* hx, double* hy, double* hxy) {
+  *hx = (CX[k] != 0 ? 1.0 : 0.0) - kCs2;
+  *hy = (CY[k] != 0 ? 1.0 : 0.0) - kCs2;
+  *hxy = 2.0 * CX[k] * CY[k];
+}
+
+// Accumulate into lm the adjoint of the node's moments from the adjoints lf,
+// lg of its populations f_k and gd_k (the transposed rebuild Jacobian).
+__device__ __forceinline__ void rebuild_adjoint(int k, const Mom& m, double lf, double lg,
+                                                Mom* lm) {
+  const double cu = CX[k] * m.u + CY[k] * m.v;
+  const double p = 3.0 * cu + 4.5 * cu * cu - 1.5 * (m.u * m.u + m.v * m.v);
+  const double dpu = 3.0 * CX[k] + 9.0 * cu * CX[k] - 3.0 * m.u;
+  const double dpv = 3.0 * CY[k] + 9.0 * cu * CY[k] - 3.0 * m.v;
+  const double w = kW[k], cf = coef_of(k);
+  double hx, hy, hxy;
+  h_coef(k, &hx, &hy, &hxy);
+  lm->rho += lf * w * (1.0 + p);
+  lm->th += lg * w * (1.0 + p);
+  const double tt = kTcrit + m.th;
+  lm->u += (lf * m.rho + lg * tt) * w * dpu;
+  lm->v += (lf * m.rho + lg * tt) * w * dpv;
+  lm->pf[0] += lf * cf * hx;
+  lm->pf[1] += lf * cf * hy;
+  lm->pf[2] += lf * cf * hxy;
+  lm->pg[0] += lg * cf * hx;
+  lm->pg[1] += lg * cf * hy;
+  lm->pg[2] += lg * cf * hxy;
+}
+
+// d ghost_gdev / dT for direction k at velocity (u, v)
+__device__ __forceinline__ double ghost_dT(int k, double u, double v) {
+  return kW[k] * (1.0 + eq_poly(k, u, v));
+}
+
+// Pulled sums and taps of step n at node (i, j) from the state before the
+// step, with the boundary folding of the forward kernels (mom_edge_nodes).
+template <bool EQF>
+__device__ __forceinline__ void pull_step(const AdjArgs& a, int i, int j, PullSums* S,
+                                          double tap[9]) {
+  const int nx = a.gm.nx, ny = a.gm.ny;
+  const int64_t P = a.gm.pitch;
+  const int64_t q = (int64_t)j * P + i;
+  const StatePtrs& s = a.sprev;
+  using V4 = typename Vec<double>::V4;
+  pull_zero(*S);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int si = i - CX[k], sj = j - CY[k];
+    const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
+    double fk = 0.0, gk;
+    const int64_t h = (int64_t)sjc * P + si;
+    if (si >= 1 && si <= nx) {
+      rebuild<double, EQF>(k, s.a[h], static_cast<const V4*>(s.b)[h],
+                           c_at<double, EQF>(s.c, h), &fk, &gk);
+    } else {
+      gk = ghost_gdev(k, si == 0 ? a.c.tin : s.te[sjc], a.uh[h], a.vh[h]);
+    }
+    if (si < 1 || si > nx || sj < 1 || sj > ny) {
+      if (k == 6 && si == nx + 1 && sj == 0)
+        fk = a.c.frozen6;
+      else if (k == 7 && si == nx + 1 && sj == ny + 1)
+        fk = a.c.frozen7;
+      else
+        fk = rebuild_f<double, EQF>(OPP[k], s.a[q], static_cast<const V4*>(s.b)[q],
+                                    c_at<double, EQF>(s.c, q));
+    }
+    pull_add(*S, k, fk, gk);
+  }
+  const int64_t rn = (int64_t)(j < ny ? j + 1 : ny) * P;
+  const int64_t rs = (int64_t)(j > 1 ? j - 1 : 1) * P;
+  const int64_t r0 = (int64_t)j * P;
+  const double* ph = s.phi;
+  tap[0] = ph[q];
+  tap[1] = ph[r0 + i + 1];
+  tap[2] = ph[rn + i];
+  tap[3] = ph[r0 + i - 1];
+  tap[4] = ph[rs + i];
+  tap[5] = ph[rn + i + 1];
+  tap[6] = ph[rn + i - 1];
+  tap[7] = ph[rs + i - 1];
+  tap[8] = ph[rs + i + 1];
+}
+
+// Reverse of core_forward (bgk_mom.h) and of the state it stores.  lo: the
+// adjoint of the node's stored moments after the step (rho, T - T_crit, u,
+// v, Pf, Pg) and of its phi; ldT2 an extra adjoint on the pulled T (the
+// averaged T of moments.f90 seeds half of its adjoint here).  Returns the
+// adjoints of the pulled sums in lS, of the tap sums in lkp, lkm and of the
+// centre tap (direct terms) in lpc; adds parameter adjoints to g[3].
+__device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S,
+                                              const double tap[9], const Core& o, const Mom& lo,
+                                              double lphi_o, double ldT2_in, PullSums* lS,
+                                              double* lkp, double* lkm, double* lpc,
+                                              double g[3]) {
+  double lrho = lo.rho, lu = lo.u, lv = lo.v, lphi = lphi_o;
+  double ldT2 = ldT2_in;
+  // Pg = (1 - omega_g) Qg, Pf = (1 - omega_f) Qf
+  const double omg1 = 1.0 - o.omega_g, omf1 = 1.0 - c.omega_f;
+  const double lqgxx = omg1 * lo.pg[0], lqgyy = omg1 * lo.pg[1], lqgxy = omg1 * lo.pg[2];
+  const double lomg = -(o.qgxx * lo.pg[0] + o.qgyy * lo.pg[1] + o.qgxy * lo.pg[2]);
+  const double lqfxx = omf1 * lo.pf[0], lqfyy = omf1 * lo.pf[1], lqfxy = omf1 * lo.pf[2];
+  // Qg = S_g - cs2 dT2 - T u u
+  ldT2 += -kCs2 * (lqgxx + lqgyy);
+  double lT = -(o.u * o.u * lqgxx + o.v * o.v * lqgyy + o.u * o.v * lqgxy);
+  lu += -2.0 * o.T * o.u * lqgxx - o.T * o.v * lqgxy;
+  lv += -2.0 * o.T * o.v * lqgyy - o.T * o.u * lqgxy;
+  // Qf = S_f - cs2 rho - rho u u
+  lrho += -(kCs2 + o.u * o.u) * lqfxx - (kCs2 + o.v * o.v) * lqfyy - o.u * o.v * lqfxy;
+  lu += -2.0 * o.rho * o.u * lqfxx - o.rho * o.v * lqfxy;
+  lv += -2.0 * o.rho * o.v * lqfyy - o.rho * o.u * lqfxy;
+  // omega_g(phi) (conductivity-ratio extension)
+  if (c.k_ratio != 1.0) {
+    const double fl = 0.5 * (1.0 + o.phi);
+    const double domg_dalpha = -o.omega_g * o.omega_g / kCs2;
+    lphi += lomg * domg_dalpha * c.alpha_l * (1.0 - c.k_ratio) * 0.5;
+    g[1] += lomg * domg_dalpha * c.alpha_l * (1.0 - fl);
+  }
+  // u = u2 + fx s, v = v2 + fy s, s = irho / omega_f
+  const double s = o.irho * c.inv_omega_f;
+  const double fx = -(o.u2 * (1.0 - o.phi) * 0.5);
+  const double gb = c.grav * c.beta;
+  const double fy0 = o.rho * gb * o.T * (1.0 + o.phi * 0.5);
+  const double fy = fy0 - o.v2 * (1.0 - o.phi) * 0.5;
+  double lu2 = lu, lv2 = lv;
+  const double lfx = s * lu, lfy = s * lv;
+  double lirho = (fx * lu + fy * lv) * c.inv_omega_f;
+  // fy = fy0 - v2 (1 - phi)/2, fx = -u2 (1 - phi)/2
+  lv2 += -(1.0 - o.phi) * 0.5 * lfy;
+  lphi += o.v2 * 0.5 * lfy;
+  lu2 += -(1.0 - o.phi) * 0.5 * lfx;
+  lphi += o.u2 * 0.5 * lfx;
+  // fy0 = rho g beta T (1 + phi/2)
+  lrho += gb * o.T * (1.0 + o.phi * 0.5) * lfy;
+  lT += o.rho * gb * (1.0 + o.phi * 0.5) * lfy;
+  lphi += o.rho * gb * o.T * 0.5 * lfy;
+  // stored T - T_crit = dT; T = T_crit + dT; dT = dT2 - dt_st (phi - pc)/2
+  const double ldT = lT + lo.th;
+  ldT2 += ldT;
+  lphi += -c.dt_st * 0.5 * ldT;
+  double lpcv = c.dt_st * 0.5 * ldT;
+  g[0] += -(o.phi - o.pc) * 0.5 * ldT * (-c.dt_st * c.inv_stefan);  // dt_st = delta_T / St
+  // phi = pc + 0.5 kpr (1 - pc) - 0.5 kmr (1 + pc), kpr = kp/8 rp, kmr = km/8 rm
+  const double rp = 1.0 - o.rm;
+  const double kpr = o.kp_sum * 0.125 * rp, kmr = o.km_sum * 0.125 * o.rm;
+  const double lkpr = 0.5 * (1.0 - o.pc) * lphi, lkmr = -0.5 * (1.0 + o.pc) * lphi;
+  lpcv += lphi * (1.0 - 0.5 * kpr - 0.5 * kmr);
+  *lkp = rp * 0.125 * lkpr;
+  *lkm = o.rm * 0.125 * lkmr;
+  const double lrm = o.km_sum * 0.125 * lkmr - o.kp_sum * 0.125 * lkpr;
+  // rm = 1/(1 + exp(dT2 * 2/T_width))
+  ldT2 += -o.rm * rp * c.two_over_tw * lrm;
+  *lpc = lpcv;
+  // u2 = jx irho, v2 = jy irho, irho = 1/rho
+  lS->jx = o.irho * lu2;
+  lS->jy = o.irho * lv2;
+  lirho += S.jx * lu2 + S.jy * lv2;
+  lrho += -o.irho * o.irho * lirho;
+  lS->rho = lrho;
+  lS->fxx = lqfxx;
+  lS->fyy = lqfyy;
+  lS->fxy = lqfxy;
+  lS->dT = ldT2;
+  lS->gxx = lqgxx;
+  lS->gyy = lqgyy;
+  lS->gxy = lqgxy;
+}
+
+template <bool EQF>
+__global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
+  const int nx = a.gm.nx, ny = a.gm.ny;
+  const int64_t P = a.gm.pitch;
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  double g[3] = {0.0, 0.0, 0.0};
+  if (i <= nx && j <= ny) {
+    const int64_t q = (int64_t)j * P + i;
+    const Mom m = load_mom<EQF>(a.sn, q);
+    Mom lm = {};
+    double lphi = 0.0;
+    double lTrep = 0.0;
+    if (a.has_next) {
+      const AdjPlanes& L = a.next;
+      // ---- reverse of step n+1's pulls from this node
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const int xi = i + CX[k], xj = j + CY[k];
+        if (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny) {
+          const int64_t x = (int64_t)xj * P + xi;
+          rebuild_adjoint(k, m, lam_f(L, k, x), lam_g(L, k, x), &lm);
+        }
+        // this node's own pull of direction k from a ghost: bounce-back of
+        // its opposite population (not for the two frozen corner ghosts)
+        const int si = i - CX[k], sj = j - CY[k];
+        if (si < 1 || si > nx || sj < 1 || sj > ny) {
+          const bool frozen = (k == 6 && si == nx + 1 && sj == 0) ||
+                              (k == 7 && si == nx + 1 && sj == ny + 1);
+          if (!frozen) rebuild_adjoint(OPP[k], m, lam_f(L, k, q), 0.0, &lm);
+          // west ghost: equilibrium at the hot-wall temperature
+          if (si == 0) {
+            const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
+            const int64_t h = (int64_t)sjc * P;
+            g[2] += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(L, k, q);
+          }
+        }
+      }
+      // ghost rows of g: row 0 is a copy of row 1, row Ny+1 of row Ny
+      if (j == 1) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+          if (CY[k] > 0) {
+            const int xi = i + CX[k];
+            if (xi >= 1 && xi <= nx) {
+              const int64_t x = (int64_t)1 * P + xi;
+              rebuild_adjoint(k, m, 0.0, lam_g(L, k, x), &lm);
+            }
+          }
+      }
+      if (j == ny) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+          if (CY[k] < 0) {
+            const int xi = i + CX[k];
+            if (xi >= 1 && xi <= nx) {
+              const int64_t x = (int64_t)ny * P + xi;
+              rebuild_adjoint(k, m, 0.0, lam_g(L, k, x), &lm);
+            }
+          }
+      }
+      // east ghost: equilibrium at this node's averaged T of step n
+      if (i == nx) {
+        for (int xj = j - 1; xj <= j + 1; ++xj) {
+          if (xj < 1 || xj > ny) continue;
+          const int64_t x = (int64_t)xj * P + nx;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            if (CX[k] >= 0) continue;  // directions 3, 6, 7 pull from column Nx+1
+            const int sj = xj - CY[k];
+            const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
+            if (sjc != j) continue;
+            const int64_t h = (int64_t)j * P + nx + 1;
+            lTrep += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(L, k, x);
+          }
+        }
+      }
+      // ---- phi of step n as a tap of step n+1 (C, E, N, W, S, NE, NW, SW, SE)
+      constexpr int DX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+      constexpr int DY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+      const double ph = a.sn.phi[q];
+      const double aa = 0.5 * (1.0 + ph), bb = 0.5 * (1.0 - ph);
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        // nodes x whose tap t reads this node: x = y - d, and in rows 1 / Ny
+        // also those whose tap t reads the ghost row copied from this node
+        int xi = i - DX[t], xj = j - DY[t];
+        if (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny) {
+          const int64_t x = (int64_t)xj * P + xi;
+          lphi += aa * L.v[L_KP][x] - bb * L.v[L_KM][x];
+        }
+        if ((j == 1 && DY[t] < 0) || (j == ny && DY[t] > 0)) {
+          xi = i - DX[t];
+          xj = j;
+          if (xi >= 1 && xi <= nx) {
+            const int64_t x = (int64_t)xj * P + xi;
+            lphi += aa * L.v[L_KP][x] - bb * L.v[L_KM][x];
+          }
+        }
+      }
+      lphi += L.v[L_PC][q];
+    }
+    if (a.core) {
+      // ---- loss seed at a probe node (averaged T after step n)
+      if (a.seeds) {
+        const int p = a.probe_idx[q];
+        if (p >= 0) lTrep += a.seeds[p];
+      }
+      // averaged T = T_crit + (dT + dT2)/2: half on the stored dT, half on dT2
+      lm.th += 0.5 * lTrep;
+      // ---- reverse of step n at this node
+      PullSums S;
+      double tap[9];
+      pull_step<EQF>(a, i, j, &S, tap);
+      const Core o = core_forward(a.c, S, tap);
+      PullSums lS;
+      double lkp, lkm, lpc;
+      core_backward(a.c, S, tap, o, lm, lphi, 0.5 * lTrep, &lS, &lkp, &lkm, &lpc, g);
+      double* const* v = a.cur.v;
+      v[L_RHO][q] = lS.rho;
+      v[L_JX][q] = lS.jx;
+      v[L_JY][q] = lS.jy;
+      v[L_FXX][q] = lS.fxx;
+      v[L_FYY][q] = lS.fyy;
+      v[L_FXY][q] = lS.fxy;
+      v[L_DT][q] = lS.dT;
+      v[L_GXX][q] = lS.gxx;
+      v[L_GYY][q] = lS.gyy;
+      v[L_GXY][q] = lS.gxy;
+      v[L_KP][q] = lkp;
+      v[L_KM][q] = lkm;
+      v[L_PC][q] = lpc;
+    } else if (i <= a.interf) {
+      // initial state: T = t_hot in the liquid strip (initialization.f90:15-26)
+      g[2] += lm.th;
+    }
+  }
+  // ---- parameter adjoints: workgroup sum into this workgroup's own slot
+  __shared__ double red[3][TX * TY];
+  const int t = threadIdx.y * TX + threadIdx.x;
+  for (int c = 0; c < 3; ++c) red[c][t] = g[c];
+  __syncthreads();
+  for (int w = TX * TY / 2; w > 0; w >>= 1) {
+    if (t < w)
+      for (int c = 0; c < 3; ++c) red[c][t] += red[c][t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    for (int c = 0; c < 3; ++c) a.partial[3 * b + c] += red[c][0];
+  }
+}
+
+// Deterministic sum of the per-workgroup slots.
+__global__ void k_adj_sum(const double* partial, int n, double* out) {
+  __shared__ double red[3][256];
+  const int t = threadIdx.x;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int b = t; b < n; b += 256)
+    for (int c = 0; c < 3; ++c) s[c] += partial[3 * b + c];
+  for (int c = 0; c < 3; ++c) red[c][t] = s[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+      for (int c = 0; c < 3; ++c) red[c][t] += red[c][t + w];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int c = 0; c < 3; ++c) out[c] = red[c][0];
+}
+
+// probe index plane: -1 everywhere (k_probe_clear), then p at probe p's
+// node (k_probe_set, a second launch on the same stream)
+__global__ void k_probe_clear(int* idx, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) idx[q] = -1;
+}
+
+__global__ void k_probe_set(int* idx, const int* pij, int np, int64_t pitch) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) idx[(int64_t)pij[2 * p + 1] * pitch + pij[2 * p]] = p;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+
+struct AdjCtx {
+  int nsteps = 0, seg = 0, np = 0, nckpt = 0;
+  void* pool = nullptr;
+  size_t bytes = 0;
+  std::vector<StatePtrs> slot;  // seg + 1 states of a segment
+  std::vector<StatePtrs> ckpt;  // checkpoints at steps 0, seg, 2 seg, ...
+  AdjPlanes lam[2];
+  int* probe_idx = nullptr;
+  int* probe_ij = nullptr;
+  double* probe_T = nullptr;  // nsteps x np
+  double* seeds = nullptr;    // nsteps x np
+  double* macro[4] = {};      // scratch macroscopic planes for recomputed steps
+  double* partial = nullptr;
+  double* out3 = nullptr;
+  int nblk = 0;
+  bool forward_done = false;
+};
+
+namespace {
+
+size_t c_bytes(const bgk_handle* h) { return h->mom_eqf ? sizeof(double) : sizeof(double4); }
+
+struct Bump {
+  char* base;
+  size_t off;
+  template <typename T>
+  T* take(size_t n) {
+    off = (off + 255) / 256 * 256;
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += n * sizeof(T);
+    return p;
+  }
+};
+
+void carve_state(Bump& b, const bgk_handle* h, StatePtrs* s) {
+  const size_t n = (size_t)h->plane_elems;
+  s->a = b.take<double2>(n);
+  s->b = b.take<double4>(n);
+  s->c = b.take<char>(n * c_bytes(h));
+  s->phi = b.take<double>(n);
+  s->te = b.take<double>((size_t)h->ny + 2);
+}
+
+size_t carve_ctx(bgk_handle* h, AdjCtx* a, char* base) {
+  Bump b{base, 0};
+  const size_t n = (size_t)h->plane_elems;
+  a->slot.assign((size_t)a->seg + 1, StatePtrs{});
+  a->ckpt.assign((size_t)a->nckpt, StatePtrs{});
+  for (auto& s : a->slot) carve_state(b, h, &s);
+  for (auto& s : a->ckpt) carve_state(b, h, &s);
+  for (int l = 0; l < 2; ++l)
+    for (int v = 0; v < 13; ++v) a->lam[l].v[v] = b.take<double>(n);
+  a->probe_idx = b.take<int>(n);
+  a->probe_ij = b.take<int>(2 * (size_t)std::max(a->np, 1));
+  a->probe_T = b.take<double>((size_t)a->nsteps * std::max(a->np, 1));
+  a->seeds = b.take<double>((size_t)a->nsteps * std::max(a->np, 1));
+  for (int m = 0; m < 4; ++m) a->macro[m] = b.take<double>(n);
+  a->partial = b.take<double>(3 * (size_t)a->nblk);
+  a->out3 = b.take<double>(3);
+  return b.off + 256;
+}
+
+hipError_t copy_state(bgk_handle* h, const StatePtrs& dst, const double2* a, const void* bb,
+                      const void* c, const double* phi, const double* te) {
+  const size_t n = (size_t)h->plane_elems;
+  hipError_t e = hipMemcpyAsync(dst.a, a, n * sizeof(double2), hipMemcpyDeviceToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dst.b, bb, n * sizeof(double4), hipMemcpyDeviceToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dst.c, c, n * c_bytes(h), hipMemcpyDeviceToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dst.phi, phi, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dst.te, te, ((size_t)h->ny + 2) * sizeof(double), hipMemcpyDeviceToDevice,
+                       h->stream);
+  return e;
+}
+
+hipError_t launch_adjoint(bgk_handle* h, AdjCtx* x, const StatePtrs& sn, const StatePtrs& sprev,
+                          int lam_next, int has_next, int core, const double* seeds) {
+  AdjArgs a;
+  a.sn = sn;
+  a.sprev = sprev;
+  a.uh = h->u;
+  a.vh = h->v;
+  a.next = x->lam[lam_next];
+  a.cur = x->lam[1 - lam_next];
+  a.has_next = has_next;
+  a.core = core;
+  a.probe_idx = x->probe_idx;
+  a.seeds = seeds;
+  a.partial = x->partial;
+  a.c = h->c;
+  a.gm = h->geom;
+  a.interf = h->nx / 40;
+  dim3 grd((unsigned)((h->nx + TX - 1) / TX), (unsigned)((h->ny + TY - 1) / TY));
+  if (h->mom_eqf)
+    hipLaunchKernelGGL(k_adjoint<true>, grd, dim3(TX, TY), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_adjoint<false>, grd, dim3(TX, TY), 0, h->stream, a);
+  return hipGetLastError();
+}
+
+// one forward step from state `in` to state `out` (explicit buffers)
+hipError_t fwd_step(bgk_handle* h, const StatePtrs& in, const StatePtrs& out, double* probe_out,
+                    const int* probe_idx, bool write_all, double* const* macro) {
+  mom::MomStep st;
+  st.in = mom::MomIn{in.a, in.b, in.c};
+  st.out = mom::MomOut{out.a, out.b, out.c};
+  st.pa = in.phi;
+  st.pb = out.phi;
+  st.te_in = in.te;
+  st.te_out = out.te;
+  st.write_all = write_all;
+  st.probe_idx = probe_idx;
+  st.probe_out = probe_out;
+  for (int m = 0; m < 4; ++m) st.macro[m] = macro ? macro[m] : nullptr;
+  return launch_mom_step(h, st);
+}
+
+}  // namespace
+
+void adj_free(bgk_handle* h) {
+  if (!h->adj) return;
+  if (h->adj->pool) (void)hipFree(h->adj->pool);
+  delete h->adj;
+  h->adj = nullptr;
+}
+
+hipError_t adj_setup(bgk_handle* h, int nsteps, int seg, int np, const int32_t* pi,
+                     const int32_t* pj) {
+  adj_free(h);
+  AdjCtx* x = new (std::nothrow) AdjCtx();
+  if (!x) return hipErrorOutOfMemory;
+  x->nsteps = nsteps;
+  x->seg = seg;
+  x->np = np;
+  x->nckpt = (nsteps + seg - 1) / seg;
+  x->nblk = (int)(((h->nx + TX - 1) / TX) * ((h->ny + TY - 1) / TY));
+  x->bytes = carve_ctx(h, x, nullptr);
+  hipError_t e = hipMalloc(&x->pool, x->bytes);
+  if (e != hipSuccess) {
+    delete x;
+    return e;
+  }
+  carve_ctx(h, x, static_cast<char*>(x->pool));
+  h->adj = x;
+  std::vector<int> ij(2 * (size_t)std::max(np, 1), 0);
+  for (int p = 0; p < np; ++p) {
+    ij[2 * p] = pi[p];
+    ij[2 * p + 1] = pj[p];
+  }
+  e = hipMemsetAsync(x->pool, 0, x->bytes, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(x->probe_ij, ij.data(), ij.size() * sizeof(int), hipMemcpyHostToDevice,
+                       h->stream);
+  if (e == hipSuccess) {
+    const int64_t n = h->plane_elems;
+    hipLaunchKernelGGL(k_probe_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                       x->probe_idx, n);
+    if (np > 0)
+      hipLaunchKernelGGL(k_probe_set, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, h->stream,
+                         x->probe_idx, x->probe_ij, np, h->pitch);
+    e = hipGetLastError();
+  }
+  // halo velocities for recomputed steps (static ghost columns of u, v)
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(x->macro[1], h->u, (size_t)h->plane_elems * sizeof(double),
+                       hipMemcpyDeviceToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(x->macro[2], h->v, (size_t)h->plane_elems * sizeof(double),
+                       hipMemcpyDeviceToDevice, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  return e;
+}
+
+hipError_t adj_forward(bgk_handle* h, double* probe_T_host) {
+  AdjCtx* x = h->adj;
+  // east column of the initial state (T(Nx, :) as the first BC_g reads it)
+  hipError_t e = launch_teast_prologue(h);
+  if (e != hipSuccess) return e;
+  int te = 0;  // teast[te] holds the current column
+  for (int n = 0; n < x->nsteps; ++n) {
+    const int a = h->cur, b = 1 - h->cur;
+    StatePtrs in{h->ma[a], h->mb[a], h->mc[a], h->phi[h->pc], h->teast[te]};
+    StatePtrs out{h->ma[b], h->mb[b], h->mc[b], h->phi[1 - h->pc], h->teast[1 - te]};
+    if (n % x->seg == 0) {
+      e = copy_state(h, x->ckpt[n / x->seg], in.a, in.b, in.c, in.phi, in.te);
+      if (e != hipSuccess) return e;
+    }
+    e = fwd_step(h, in, out, x->probe_T + (size_t)n * x->np, x->probe_idx, n == x->nsteps - 1,
+                 nullptr);
+    if (e != hipSuccess) return e;
+    h->cur = b;
+    h->pc = 1 - h->pc;
+    te = 1 - te;
+  }
+  // the handle's teast convention: bgk_step recomputes it from T at entry
+  if (x->np > 0)
+    e = hipMemcpyAsync(probe_T_host, x->probe_T, (size_t)x->nsteps * x->np * sizeof(double),
+                       hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) x->forward_done = true;
+  return e;
+}
+
+hipError_t adj_backward(bgk_handle* h, const double* seeds_host, double grad[3]) {
+  AdjCtx* x = h->adj;
+  hipError_t e = hipSuccess;
+  if (x->np > 0)
+    e = hipMemcpyAsync(x->seeds, seeds_host, (size_t)x->nsteps * x->np * sizeof(double),
+                       hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(x->partial, 0, 3 * (size_t)x->nblk * sizeof(double), h->stream);
+  int ln = 0;  // lam[ln] holds the adjoint of the step after the current one
+  bool has_next = false;
+  for (int s = x->nckpt - 1; s >= 0 && e == hipSuccess; --s) {
+    const int n0 = s * x->seg;
+    const int n1 = std::min(n0 + x->seg, x->nsteps);
+    const StatePtrs& ck = x->ckpt[s];
+    e = copy_state(h, x->slot[0], ck.a, ck.b, ck.c, ck.phi, ck.te);
+    for (int n = n0; n < n1 && e == hipSuccess; ++n)
+      e = fwd_step(h, x->slot[n - n0], x->slot[n - n0 + 1], nullptr, nullptr, false, x->macro);
+    for (int n = n1; n > n0 && e == hipSuccess; --n) {
+      const double* sd = x->np > 0 ? x->seeds + (size_t)(n - 1) * x->np : nullptr;
+      e = launch_adjoint(h, x, x->slot[n - n0], x->slot[n - n0 - 1], ln, has_next ? 1 : 0, 1, sd);
+      ln = 1 - ln;
+      has_next = true;
+    }
+  }
+  // the initial state's share (hot-wall temperature of the liquid strip)
+  if (e == hipSuccess)
+    e = launch_adjoint(h, x, x->slot[0], x->slot[0], ln, has_next ? 1 : 0, 0, nullptr);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, h->stream, x->partial, x->nblk, x->out3);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(grad, x->out3, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  return e;
+}
+
+}  // namespace bgk

@@ -88,7 +88,9 @@ size_t carve(bgk_handle* h, char* base) {
   Carver cv{base, 0};
   const int64_t n = h->plane_elems;
   const int prec = h->prm.precision;
-  const bool fp32 = prec == BGK_FP32_DEV, mom = prec == BGK_FP32_MOM;
+  const bool fp32 = prec == BGK_FP32_DEV;
+  const bool mom = prec == BGK_FP32_MOM || prec == BGK_FP64_MOM;
+  const bool m64 = prec == BGK_FP64_MOM;
   for (int b = 0; b < 2; ++b) {
     h->ma[b] = nullptr;
     h->mb[b] = nullptr;
@@ -99,9 +101,15 @@ size_t carve(bgk_handle* h, char* base) {
     }
     if (mom) {
       h->ma[b] = cv.plane<double2>(n);
-      h->mb[b] = cv.plane<float4>(n);
-      h->mc[b] = h->mom_eqf ? static_cast<void*>(cv.plane<float>(n))
-                            : static_cast<void*>(cv.plane<float4>(n));
+      if (m64) {
+        h->mb[b] = cv.plane<double4>(n);
+        h->mc[b] = h->mom_eqf ? static_cast<void*>(cv.plane<double>(n))
+                              : static_cast<void*>(cv.plane<double4>(n));
+      } else {
+        h->mb[b] = cv.plane<float4>(n);
+        h->mc[b] = h->mom_eqf ? static_cast<void*>(cv.plane<float>(n))
+                              : static_cast<void*>(cv.plane<float4>(n));
+      }
       continue;
     }
     for (int k = 0; k < 9; ++k) {
@@ -154,8 +162,12 @@ void set_constants(bgk_handle* h) {
   c.rho1 = p.rho1;
   c.t0 = p.t0;
   c.t_in_sol = p.t_in_sol;
-  // STEP_7/initialization.f90:9-12
-  c.tin = p.t0;
+  // STEP_7/initialization.f90:9-12 (the hot wall and the liquid strip at
+  // T0 unless the t_hot extension moves them)
+  c.tin = h->ext.t_hot > 0.0 ? h->ext.t_hot : p.t0;
+  c.t_liquid0 = c.tin;
+  c.k_ratio = h->ext.k_ratio;
+  c.alpha_l = c.cs2 * (p.tau_g - 0.5);
   const double kvisc = c.cs2 * (p.tau_f - 0.5);
   c.u0 = p.re * kvisc / p.length;
   c.v0 = 0.0;
@@ -167,6 +179,7 @@ void set_constants(bgk_handle* h) {
   c.grav = p.grav;
   c.t_width = p.t_width;
   c.two_over_tw = 2.0 / p.t_width;
+  c.inv_stefan = 1.0 / p.stefan_number;
   c.g_ref = 273.0;  // T_crit of STEP_7/phase_field.f90:16
   // The initial equilibria of the two ghosts BC_f never writes
   // (initialization.f90:22-26,76-81: rho0 right of the interface, u = v = 0
@@ -185,7 +198,7 @@ int check_handle(bgk_handle* h) {
 int need_fp64(bgk_handle* h, const char* what) {
   if (h->prm.precision != BGK_FP64)
     return fail(BGK_E_UNSUPPORTED, "%s needs an FP64 handle (this one stores %s)", what,
-                h->prm.precision == BGK_FP32_MOM ? "FP32 moments" : "FP32 deviations");
+                h->prm.precision == BGK_FP32_DEV ? "FP32 deviations" : "moments");
   return BGK_OK;
 }
 
@@ -206,7 +219,8 @@ struct FieldRef {
 int resolve(bgk_handle* h, int field, FieldRef* r) {
   const bool fp32 = h->prm.precision == BGK_FP32_DEV;
   if (field < 0 || field >= BGK_NFIELDS) return fail(BGK_E_INVALID, "field id %d", field);
-  if (h->prm.precision == BGK_FP32_MOM && field < BGK_RHO)
+  const bool mom = h->prm.precision == BGK_FP32_MOM || h->prm.precision == BGK_FP64_MOM;
+  if (mom && field < BGK_RHO)
     return fail(BGK_E_UNSUPPORTED, "a moment handle stores no populations (field %d)", field);
   if (field < BGK_FP0 || (field >= BGK_G0 && field < BGK_GP0)) {
     const bool isf = field < BGK_FP0;
@@ -280,7 +294,7 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
       p->rho0 == 0.0 || p->t0 == p->t_in_sol)
     return fail(BGK_E_INVALID, "tau_f, tau_g, stefan_number, T_width, rho0 and T0-T_in_sol "
                                "must be nonzero");
-  if (p->precision != BGK_FP64 && p->precision != BGK_FP32_DEV && p->precision != BGK_FP32_MOM)
+  if (p->precision < BGK_FP64 || p->precision > BGK_FP64_MOM)
     return fail(BGK_E_INVALID, "precision %d", p->precision);
   if (p->rounding != BGK_ROUND_STEP7 && p->rounding != BGK_ROUND_CLEAN)
     return fail(BGK_E_INVALID, "rounding %d", p->rounding);
@@ -311,8 +325,11 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
   h->small_elems = std::max<int64_t>(4096, 3 * (int64_t)(p->nx + p->ny + 4));
   h->geom = bgk::Geom{p->nx, p->ny, h->pitch};
   h->fused_ok = fused_ok;
+  h->adj = nullptr;
   h->cur = 0;
   h->pc = 0;
+  h->ext.t_hot = 0.0;
+  h->ext.k_ratio = 1.0;
   // tiling of the moment kernel: BGK_MOM_ROWS (rows per strip; default
   // about 64, or 32 for tau_f != 1, fitted to the device) and BGK_MOM_VARIANT (1..8: workgroup width 64/128/256/512 with a 4- or
   // 3-row ring; 0: chosen from Nx) override the defaults; tuning knobs,
@@ -357,6 +374,7 @@ int bgk_destroy(bgk_handle* h) {
   if (!h) return BGK_OK;
   (void)hipSetDevice(h->device);
   hipError_t e1 = hipStreamSynchronize(h->stream);
+  bgk::adj_free(h);
   hipError_t e2 = hipStreamDestroy(h->own_stream);
   hipError_t e3 = hipFree(h->pool);
   delete h;
@@ -378,9 +396,26 @@ int bgk_initialization(bgk_handle* h) {
   int rc = check_handle(h);
   if (rc) return rc;
   if (h->prm.precision == BGK_FP32_DEV) return launched(bgk::launch_initialization32(h), "initialization");
-  if (h->prm.precision == BGK_FP32_MOM)
+  if (h->prm.precision == BGK_FP32_MOM || h->prm.precision == BGK_FP64_MOM)
     return launched(bgk::launch_initialization_mom(h), "initialization");
   return launched(bgk::launch_initialization(h), "initialization");
+}
+
+int bgk_set_extensions(bgk_handle* h, const bgk_ext* e) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!e) return fail(BGK_E_INVALID, "null extensions");
+  if (!(e->t_hot >= 0.0) || !(e->k_ratio > 0.0))
+    return fail(BGK_E_INVALID, "t_hot=%g k_ratio=%g: need t_hot >= 0 and k_ratio > 0", e->t_hot,
+                e->k_ratio);
+  const bool mom = h->prm.precision == BGK_FP32_MOM || h->prm.precision == BGK_FP64_MOM;
+  const bool off = (e->t_hot == 0.0 || e->t_hot == h->prm.t0) && e->k_ratio == 1.0;
+  if (!mom && !off)
+    return fail(BGK_E_UNSUPPORTED, "physics extensions need a moment handle "
+                                   "(precision BGK_FP32_MOM or BGK_FP64_MOM)");
+  h->ext = *e;
+  set_constants(h);
+  return BGK_OK;
 }
 
 int bgk_upload(bgk_handle* h, int32_t field, const double* src, int64_t ld) {
@@ -388,7 +423,7 @@ int bgk_upload(bgk_handle* h, int32_t field, const double* src, int64_t ld) {
   if (rc) return rc;
   if (!src) return fail(BGK_E_INVALID, "null source");
   if (ld < h->nx + 2) return fail(BGK_E_INVALID, "ld=%lld < Nx+2=%d", (long long)ld, h->nx + 2);
-  if (h->prm.precision == BGK_FP32_MOM)
+  if (h->prm.precision == BGK_FP32_MOM || h->prm.precision == BGK_FP64_MOM)
     return fail(BGK_E_UNSUPPORTED, "a moment handle's state is set by bgk_initialization only");
   FieldRef r;
   if ((rc = resolve(h, field, &r))) return rc;
@@ -447,7 +482,7 @@ int bgk_step(bgk_handle* h, int32_t nsteps) {
     for (int s = 0; s < nsteps; ++s) {
       const bool last = (s == nsteps - 1);
       hipError_t e = prec == BGK_FP32_DEV   ? bgk::launch_fused_step32(h, s, last)
-                     : prec == BGK_FP32_MOM ? bgk::launch_fused_step_mom(h, s, last)
+                     : prec >= BGK_FP32_MOM ? bgk::launch_fused_step_mom(h, s, last)
                                             : bgk::launch_fused_step64(h, s, last);
       if (e != hipSuccess) return hip_fail(e, "fused step");
     }
@@ -554,6 +589,43 @@ int bgk_gather(bgk_handle* h, int32_t field, int32_t n, const int32_t* i, const 
     }
   }
   delete[] pairs;
+  return BGK_OK;
+}
+
+int bgk_adj_setup(bgk_handle* h, int32_t nsteps, int32_t seg_len, int32_t n_probes,
+                  const int32_t* pi, const int32_t* pj) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (h->prm.precision != BGK_FP64_MOM)
+    return fail(BGK_E_UNSUPPORTED, "the adjoint differentiates the FP64 moment step: create the "
+                                   "handle with precision BGK_FP64_MOM");
+  if (nsteps < 1 || seg_len < 1 || n_probes < 0 || (n_probes > 0 && (!pi || !pj)))
+    return fail(BGK_E_INVALID, "nsteps=%d seg_len=%d n_probes=%d", nsteps, seg_len, n_probes);
+  for (int32_t p = 0; p < n_probes; ++p)
+    if (pi[p] < 1 || pi[p] > h->nx || pj[p] < 1 || pj[p] > h->ny)
+      return fail(BGK_E_INVALID, "probe %d at (%d,%d) is not an interior node", p, pi[p], pj[p]);
+  hipError_t e = bgk::adj_setup(h, nsteps, seg_len, n_probes, pi, pj);
+  if (e != hipSuccess) return hip_fail(e, "adjoint setup");
+  return BGK_OK;
+}
+
+int bgk_adj_forward(bgk_handle* h, double* probe_T) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!h->adj) return fail(BGK_E_INVALID, "bgk_adj_setup first");
+  if (!probe_T) return fail(BGK_E_INVALID, "null probe record");
+  hipError_t e = bgk::adj_forward(h, probe_T);
+  if (e != hipSuccess) return hip_fail(e, "adjoint forward");
+  return BGK_OK;
+}
+
+int bgk_adj_backward(bgk_handle* h, const double* seeds, double grad[3]) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!h->adj) return fail(BGK_E_INVALID, "bgk_adj_setup first");
+  if (!seeds || !grad) return fail(BGK_E_INVALID, "null seeds or gradient");
+  hipError_t e = bgk::adj_backward(h, seeds, grad);
+  if (e != hipSuccess) return hip_fail(e, "adjoint backward");
   return BGK_OK;
 }
 

@@ -105,6 +105,11 @@ struct Consts {
   double frozen6, frozen7;
   double inv_omega_f;  // 1/omega_f, for kernels that need not round as STEP_7
   double two_over_tw;  // 2/T_width, same
+  // extensions (bgk_ext; STEP_7: t_liquid0 = T0, k_ratio = 1)
+  double t_liquid0;    // initial temperature of the liquid strip
+  double k_ratio;      // solid/liquid conductivity ratio
+  double alpha_l;      // liquid thermal diffusivity cs2 (tau_g - 1/2)
+  double inv_stefan;   // 1/stefan_number (adjoint)
 };
 
 // XCD-aware tiling of the fused kernels.  The lattice interior is cut into
@@ -233,7 +238,7 @@ __device__ __forceinline__ NodeInit initial_node(const Consts& c, const Geom& gm
                                                  int interf, int west_bc) {
   NodeInit s;
   if (i <= interf) {
-    s.T = c.t0;
+    s.T = c.t_liquid0;
     s.rho = c.rho1;
   } else {
     s.T = c.t_in_sol;

@@ -1,185 +1,51 @@
-// bgk_fused_mom.hip -- the fused time step with FP32 moment storage.
+// bgk_fused_mom.hip -- the fused time step with moment storage.
 //
-// After the regularised collision (STEP_7/collision.f90:44-150, clean
-// rounding) every population of a node is a function of ten numbers:
-//   f_k = w_k rho (1 + P_k(u,v)) + coef_k H_k : Pf,   Pf = (1 - omega_f) Pi_f^neq
-//   g_k = w_k T   (1 + P_k(u,v)) + coef_k H_k : Pg,   Pg = (1 - omega_g) Pi_g^neq
-// (P_k the equilibrium polynomial minus its leading 1, H_k = c_k c_k - cs2 I,
-// coef_k = w_k / (2 cs2^2)).  So a node stores those ten numbers instead of
-// eighteen populations, and the pull rebuilds population k of the upstream
-// node from its moments.  Layout per node, 48 bytes in three 16-byte planes:
-//   A double2 (rho, T - T_crit)           FP64
-//   B float4  (u, v, Pf_xx, Pf_yy)        FP32
-//   C float4  (Pf_xy, Pg_xx, Pg_yy, Pg_xy) FP32
-// rho and T stay FP64: the rounding of rho (not of u) is what drives the
-// velocity error of FP32 storage, since a neighbour's density error enters
-// the momentum balance through sum_k c_k w_k rho(x - c_k).  u, v and the
-// non-equilibrium moments carry their own relative precision in FP32.
+// The node state is the ten moments of bgk_mom.h (rho, T - T_crit in FP64;
+// u, v and the scaled non-equilibrium second moments in R = float or
+// double), 48 bytes per node with FP32 R (36 with omega_f = 1) instead of
+// 144 for FP32 populations, and the pull rebuilds population k of the
+// upstream node from its moments.  rho and T stay FP64: the rounding of rho
+// (not of u) is what drives the velocity error of FP32 storage, since a
+// neighbour's density error enters the momentum balance through
+// sum_k c_k w_k rho(x - c_k).  u, v and the non-equilibrium moments carry
+// their own relative precision.
 //
 // Tiling: a workgroup owns W columns and a strip of consecutive rows, and
 // walks the strip row by row.  Row j+2 is fetched from HBM while row j is
-// computed; rows j-1..j+2 live in a four-row LDS ring, so every node's
-// moments and phi are read from HBM once per step (plus the strip's two
-// boundary rows and the two halo columns).  phi_prec's 9-point stencil comes
-// from the same ring.  BC_f (no-slip) and BC_g are folded in as in
-// bgk_fused.hip: ghost rows of g and phi are the clamped rows, own opposite
-// populations for bounce-back, the two frozen corner ghosts as constants.
+// computed; rows j-1..j+2 live in an LDS ring, so every node's moments and
+// phi are read from HBM once per step (plus the strip's two boundary rows
+// and the two halo columns).  phi_prec's 9-point stencil comes from the
+// same ring.  The boundary-adjacent nodes run in extra workgroups of the
+// same launch with BC_f (no-slip) and BC_g folded in as in bgk_fused.hip:
+// ghost rows of g and phi are the clamped rows, own opposite populations
+// for bounce-back, the two frozen corner ghosts as constants.
 //
 // Arithmetic is FP64 throughout.  STEP_7's FP32 temporaries (ft, gt) cannot
 // be represented by moments, so this path follows the clean rounding policy;
 // beta follows bgk_params.rounding as everywhere.
 #include <algorithm>
 
-#include "bgk_internal.h"
+#include "bgk_mom.h"
 
 namespace bgk {
 namespace {
 
+using namespace mom;
+
 constexpr int TX = 64, TY = 4;  // block of the initialization kernel
-
-// Lattice constants as literals (STEP_7/setup.f90:79-99), so that they
-// need no scalar registers: w_k, cs2 and coef_k = (0.5 w_k)/(cs2 cs2).
-constexpr double kCs2 = 1.0 / 3.0;
-constexpr double kW[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
-                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
-constexpr double coef_of(int k) { return (0.5 * kW[k]) / (kCs2 * kCs2); }
-constexpr double kTcrit = 273.0;  // T_crit, STEP_7/phase_field.f90:16
-
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ float4 stream_load4(const float4* p) {
-  const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
-  return make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ void stream_store4(float4* p, float x, float y, float z, float w) {
-  f32x4_t v;
-  v.x = x;
-  v.y = y;
-  v.z = z;
-  v.w = w;
-  __builtin_nontemporal_store(v, reinterpret_cast<f32x4_t*>(p));
-}
 
 struct Macro {
   double *rho, *u, *v, *T;
 };
 
-// Element of the third plane.  With omega_f = 1 (tau_f = 1, the reference
-// inputs) the regularised flow collision returns the equilibrium, Pf = 0,
-// and a node needs 36 bytes: B = (u, v, Pg_xx, Pg_yy), C = Pg_xy.
-template <bool EQF>
-struct CPlane {
-  using T = float4;  // Pf_xy, Pg_xx, Pg_yy, Pg_xy
-};
-template <>
-struct CPlane<true> {
-  using T = float;  // Pg_xy
-};
-
-// Moments of one node as held in the ring.
-template <bool EQF>
-struct NodeM {
-  double2 a;  // rho, T - T_crit
-  float4 b;   // u, v, Pf_xx, Pf_yy  (EQF: u, v, Pg_xx, Pg_yy)
-  typename CPlane<EQF>::T c;
-  double phi;
-};
-
 // LDS ring of NS rows of W + 2 nodes (W columns and the two halo columns).
-template <int W, int NS, bool EQF>
+template <int W, int NS, typename R, bool EQF>
 struct Ring {
   double2 a[NS][W + 2];
-  float4 b[NS][W + 2];
-  typename CPlane<EQF>::T c[NS][W + 2];
+  typename Vec<R>::V4 b[NS][W + 2];
+  typename CType<R, EQF>::T c[NS][W + 2];
   double phi[NS][W + 2];
 };
-
-// P_k(u,v): the equilibrium polynomial of direction k minus its leading 1.
-__device__ __forceinline__ double eq_poly(int k, double u, double v) {
-  const double uv = 1.5 * (u * u + v * v);
-  switch (k) {
-    case 0: return -uv;
-    case 1: return 3.0 * u + 4.5 * (u * u) - uv;
-    case 2: return 3.0 * v + 4.5 * (v * v) - uv;
-    case 3: return -3.0 * u + 4.5 * (u * u) - uv;
-    case 4: return -3.0 * v + 4.5 * (v * v) - uv;
-    case 5: { const double s = u + v; return 3.0 * s + 4.5 * (s * s) - uv; }
-    case 6: { const double s = u - v; return -3.0 * s + 4.5 * (s * s) - uv; }
-    case 7: { const double s = u + v; return -3.0 * s + 4.5 * (s * s) - uv; }
-    default: { const double s = u - v; return 3.0 * s + 4.5 * (s * s) - uv; }
-  }
-}
-
-// H_k : P for P = (pxx, pyy, pxy), H_k = c_k c_k - cs2 I, evaluated in FP32: its operands are FP32 already and the result
-// keeps their relative precision.
-__device__ __forceinline__ float h_k32(int k, float pxx, float pyy, float pxy) {
-  constexpr float A = -1.0f / 3.0f, B = 2.0f / 3.0f;
-  switch (k) {
-    case 0: return A * (pxx + pyy);
-    case 1:
-    case 3: return B * pxx + A * pyy;
-    case 2:
-    case 4: return A * pxx + B * pyy;
-    case 5:
-    case 7: return B * (pxx + pyy) + 2.0f * pxy;
-    default: return B * (pxx + pyy) - 2.0f * pxy;
-  }
-}
-
-// Population k of a node from its moments: f_k and g_k - w_k*T_crit.
-__device__ __forceinline__ void rebuild(int k, const double2 a, const float4 b, const float4 cc,
-                                        double* fk, double* gdk) {
-  const double p = eq_poly(k, (double)b.x, (double)b.y);
-  const double hf = (double)h_k32(k, b.z, b.w, cc.x);
-  const double hg = (double)h_k32(k, cc.y, cc.z, cc.w);
-  *fk = kW[k] * a.x * (1.0 + p) + coef_of(k) * hf;
-  *gdk = kW[k] * (a.y + (kTcrit + a.y) * p) + coef_of(k) * hg;
-}
-__device__ __forceinline__ void rebuild(int k, const double2 a, const float4 b, const float cc,
-                                        double* fk, double* gdk) {
-  const double p = eq_poly(k, (double)b.x, (double)b.y);
-  const double hg = (double)h_k32(k, b.z, b.w, cc);
-  *fk = kW[k] * a.x * (1.0 + p);
-  *gdk = kW[k] * (a.y + (kTcrit + a.y) * p) + coef_of(k) * hg;
-}
-
-__device__ __forceinline__ double rebuild_f(int k, const double2 a, const float4 b,
-                                            const float4 cc) {
-  const double p = eq_poly(k, (double)b.x, (double)b.y);
-  const double hf = (double)h_k32(k, b.z, b.w, cc.x);
-  return kW[k] * a.x * (1.0 + p) + coef_of(k) * hf;
-}
-__device__ __forceinline__ double rebuild_f(int k, const double2 a, const float4 b, float) {
-  return kW[k] * a.x * (1.0 + eq_poly(k, (double)b.x, (double)b.y));
-}
-
-// Equilibrium deviation of a boundary ghost, g_k - w_k*T_crit.
-__device__ __forceinline__ double ghost_gdev(int k, double T, double u, double v) {
-  return kW[k] * (T - kTcrit) + kW[k] * T * eq_poly(k, u, v);
-}
-
-struct MomIn {
-  const double2* a;
-  const float4* b;
-  const void* c;  // float4 or float (CPlane)
-};
-struct MomOut {
-  double2* a;
-  float4* b;
-  void* c;
-};
-
-template <bool EQF>
-__device__ __forceinline__ typename CPlane<EQF>::T c_at(const void* c, int64_t q) {
-  return static_cast<const typename CPlane<EQF>::T*>(c)[q];
-}
-template <bool EQF>
-__device__ __forceinline__ typename CPlane<EQF>::T c_stream(const void* c, int64_t q) {
-  if constexpr (EQF)
-    return __builtin_nontemporal_load(static_cast<const float*>(c) + q);
-  else
-    return stream_load4(static_cast<const float4*>(c) + q);
-}
 
 // Row strips: workgroup b runs on XCD b % 8; all column segments of a strip
 // go to one XCD (its halo columns are L2 hits) and the eight XCDs work on
@@ -191,179 +57,81 @@ struct StripGrid {
   int nmain;   // workgroups of the strips (padded to a multiple of 8 strips)
 };
 
-template <bool EQF>
-__device__ __forceinline__ NodeM<EQF> load_node(const MomIn& in, const double* __restrict__ pa,
-                                                const Geom& gm, int ic, int jr) {
-  NodeM<EQF> n;
+template <typename R, bool EQF>
+__device__ __forceinline__ Node<R, EQF> load_node(const MomIn& in, const double* __restrict__ pa,
+                                                  const Geom& gm, int ic, int jr) {
+  Node<R, EQF> n;
   n.a = make_double2(0.0, 0.0);
-  n.b = make_float4(0.f, 0.f, 0.f, 0.f);
-  n.c = typename CPlane<EQF>::T{};
+  n.b = make4<R>(R(0), R(0), R(0), R(0));
+  n.c = typename CType<R, EQF>::T{};
   n.phi = 0.0;
   const int64_t q = (int64_t)jr * gm.pitch + ic;
   if (ic >= 1 && ic <= gm.nx) {
     n.a = stream_load(in.a + q);
-    n.b = stream_load4(in.b + q);
-    n.c = c_stream<EQF>(in.c, q);
+    n.b = nt_load4<R>(static_cast<const typename Vec<R>::V4*>(in.b) + q);
+    n.c = c_stream<R, EQF>(in.c, q);
   }
   if (ic >= 0 && ic <= gm.nx + 1) n.phi = pa[q];
   return n;
 }
 
-template <int W, int NS, bool EQF>
-__device__ __forceinline__ void put(Ring<W, NS, EQF>& L, int r, int s, const NodeM<EQF>& n) {
+template <int W, int NS, typename R, bool EQF>
+__device__ __forceinline__ void put(Ring<W, NS, R, EQF>& L, int r, int s, const Node<R, EQF>& n) {
   L.a[r][s] = n.a;
   L.b[r][s] = n.b;
   L.c[r][s] = n.c;
   L.phi[r][s] = n.phi;
 }
 
-// Phase-field update of STEP_7/phase_field.f90:24-52 with the reaction rates
-// in closed form: 0.5 (1 -/+ tanh x) = 1 / (1 + exp(+/-2x)), x = (T - 273) /
-// T_width.  One exp and one division instead of the branchy glibc-exact tanh
-// of the FP64 kernels (which this path, rounding as clean FP64, does not
-// need to reproduce bit for bit); the absolute error of the rates is a few
-// ulp of 1, like that of 0.5 (1 -/+ tanh x).  Evaluated at every node
-// without a branch: inside one phase the reaction terms are multiplied by
-// exact zeros anyway, and a branch here splits the node's code and costs
-// ~70 registers.
-__device__ __forceinline__ double phase_update_exp(const Consts& c, double T,
-                                                   const double tap[9]) {
-  double a = 0.5 * (1.0 + tap[0]);
-  double kp = a * a;
-  double b = 0.5 * (1.0 - tap[0]);
-  double km = b * b;
-#pragma unroll
-  for (int m = 1; m < 9; ++m) {
-    a = 0.5 * (1.0 + tap[m]);
-    kp = kp + a * a;
-    b = 0.5 * (1.0 - tap[m]);
-    km = km + b * b;
-  }
-  const double pc = tap[0];
-  const double react_minus = 1.0 / (1.0 + exp((T - kTcrit) * c.two_over_tw));
-  const double react_plus = 1.0 - react_minus;
-  kp = (kp * 0.125) * react_plus;
-  km = (km * 0.125) * react_minus;
-  return pc + (0.5 * kp * (1.0 - pc) - 0.5 * km * (1.0 + pc));
-}
-
-// Moments of the pulled populations of a node: zeroth, first and second
-// moments of f, and of gd = g - w*T_crit (its first moment is not needed).
-struct PullSums {
-  double rho, jx, jy, fxx, fyy, fxy;
-  double dT, gxx, gyy, gxy;
-};
-
-__device__ __forceinline__ void pull_zero(PullSums& S) {
-  S.rho = S.jx = S.jy = S.fxx = S.fyy = S.fxy = 0.0;
-  S.dT = S.gxx = S.gyy = S.gxy = 0.0;
-}
-
-// Add population k (f_k, gd_k) to the sums.
-__device__ __forceinline__ void pull_add(PullSums& S, int k, double fk, double gk) {
-  S.rho += fk;
-  S.dT += gk;
-  if (CX[k] != 0) {
-    S.jx += CX[k] > 0 ? fk : -fk;
-    S.fxx += fk;
-    S.gxx += gk;
-  }
-  if (CY[k] != 0) {
-    S.jy += CY[k] > 0 ? fk : -fk;
-    S.fyy += fk;
-    S.gyy += gk;
-  }
-  if (CX[k] * CY[k] > 0) {
-    S.fxy += fk;
-    S.gxy += gk;
-  } else if (CX[k] * CY[k] < 0) {
-    S.fxy -= fk;
-    S.gxy -= gk;
-  }
-}
-
-// momentsPre, phase field, forces and collision of node q from the moments
-// of its pulled populations and its phi_prec taps (C, E, N, W, S, NE, NW,
-// SW, SE); stores the node's moments and phi.  Returns what the averaged
-// moments of STEP_7/moments.f90:11-25 need: rho, the averaged T (Tav), u, v.
-struct NodeOut {
-  double rho, Tav, uav, vav;
-};
-
-template <bool EQF>
+// Node update from its pulled moments and phi_prec taps (C, E, N, W, S, NE,
+// NW, SW, SE); stores the node's moments and phi, and T (averaged as
+// STEP_7/moments.f90:11-25) at a probe node.  Returns the averaged moments.
+template <typename R, bool EQF>
 __device__ __forceinline__ NodeOut update_node(const Consts& c, const PullSums& S,
                                                const double tap[9], const MomOut& out,
-                                               double* __restrict__ pb, int64_t q) {
-  // ---- momentsPre (STEP_7/momentsPre.f90:8-21)
-  const double rho = S.rho;
-  double dT = S.dT;
-  const double irho = 1.0 / rho;
-  double u = S.jx * irho;
-  double v = S.jy * irho;
-  const double dT2 = dT, u2 = u, v2 = v;
-
-  // ---- phase field (STEP_7/phase_field.f90:24-52)
-  const double phi = phase_update_exp(c, kTcrit + dT, tap);
-  const double pc = tap[0];
-
-  // ---- latent heat and forces (STEP_7/forces.f90:18-26)
-  dT = dT - (c.dt_st * (phi - pc) * 0.5);
-  const double T = kTcrit + dT;
-  {
-    double fy = rho * c.grav * c.beta * T * (1.0 + phi * 0.5);
-    const double fx = -(u * (1.0 - phi) * 0.5);
-    fy = fy - v * (1.0 - phi) * 0.5;
-    const double s = irho * c.inv_omega_f;
-    u = u + fx * s;
-    v = v + fy * s;
-  }
-
-  // ---- collision (STEP_7/collision.f90:44-92) in moment form: the
-  // non-equilibrium second moments of the pulled populations about the
-  // equilibria at (rho, u, v) and (T, u, v); for the second-order D2Q9
-  // equilibrium sum_k H_k feq_k = rho u u.
-  const double omg1 = 1.0 - c.omega_g;
-  const double pgxx = omg1 * (S.gxx - kCs2 * dT2 - T * u * u);
-  const double pgyy = omg1 * (S.gyy - kCs2 * dT2 - T * v * v);
-  const double pgxy = omg1 * (S.gxy - T * u * v);
-  const float us = (float)u, vs = (float)v;
-  stream_store(out.a + q, rho, dT);
+                                               double* __restrict__ pb, int64_t q,
+                                               const int* __restrict__ probe_idx,
+                                               double* __restrict__ probe_out) {
+  const Core o = core_forward(c, S, tap);
+  const R us = (R)o.u, vs = (R)o.v;
+  const R pgxx = (R)((1.0 - o.omega_g) * o.qgxx);
+  const R pgyy = (R)((1.0 - o.omega_g) * o.qgyy);
+  const R pgxy = (R)((1.0 - o.omega_g) * o.qgxy);
+  stream_store(out.a + q, o.rho, o.dT);
+  using V4 = typename Vec<R>::V4;
   if constexpr (EQF) {
-    stream_store4(out.b + q, us, vs, (float)pgxx, (float)pgyy);
-    __builtin_nontemporal_store((float)pgxy, static_cast<float*>(out.c) + q);
+    nt_store4<R>(static_cast<V4*>(out.b) + q, us, vs, pgxx, pgyy);
+    __builtin_nontemporal_store(pgxy, static_cast<R*>(out.c) + q);
   } else {
     const double omf1 = 1.0 - c.omega_f;
-    const double pfxx = omf1 * (S.fxx - kCs2 * rho - rho * u * u);
-    const double pfyy = omf1 * (S.fyy - kCs2 * rho - rho * v * v);
-    const double pfxy = omf1 * (S.fxy - rho * u * v);
-    stream_store4(out.b + q, us, vs, (float)pfxx, (float)pfyy);
-    stream_store4(static_cast<float4*>(out.c) + q, (float)pfxy, (float)pgxx, (float)pgyy,
-                  (float)pgxy);
+    nt_store4<R>(static_cast<V4*>(out.b) + q, us, vs, (R)(omf1 * o.qfxx), (R)(omf1 * o.qfyy));
+    nt_store4<R>(static_cast<V4*>(out.c) + q, (R)(omf1 * o.qfxy), pgxx, pgyy, pgxy);
   }
-  stream_store(&pb[q], phi);
-
-  // ---- moments of the stored state (STEP_7/moments.f90:11-25): sum_k f_k =
-  // rho, sum_k g_k = T and sum_k c_k f_k / rho = u exactly for the rebuilt
-  // populations
-  NodeOut o;
-  o.rho = rho;
-  o.Tav = kTcrit + (dT + dT2) * 0.5;
-  o.uav = ((double)us + u2) * 0.5;
-  o.vav = ((double)vs + v2) * 0.5;
-  return o;
+  stream_store(&pb[q], o.phi);
+  NodeOut r;
+  r.rho = o.rho;
+  r.Tav = kTcrit + (o.dT + o.dT2) * 0.5;
+  r.uav = ((double)us + o.u2) * 0.5;
+  r.vav = ((double)vs + o.v2) * 0.5;
+  if (probe_out) {
+    const int p = probe_idx[q];
+    if (p >= 0) probe_out[p] = r.Tav;
+  }
+  return r;
 }
 
 // Interior nodes (1 < i < Nx, 1 < j < Ny) by row strips through the LDS
 // ring.  W columns per workgroup, NS rows in the ring (4: one barrier per
 // row; 3: two, for less LDS).  Workgroups from sg.nmain on serve the
 // boundary-adjacent nodes (mom_edge_nodes).
-template <int W, int NS, bool EQF>
+template <int W, int NS, typename R, bool EQF>
 __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
                                              const double* __restrict__ pa,
                                              double* __restrict__ pb, const Macro& m,
                                              const Consts& c, const Geom& gm, const StripGrid& sg,
-                                             int write_all) {
-  __shared__ Ring<W, NS, EQF> L;
+                                             int write_all, const int* __restrict__ probe_idx,
+                                             double* __restrict__ probe_out) {
+  __shared__ Ring<W, NS, R, EQF> L;
   const int nx = gm.nx, ny = gm.ny;
   const int64_t P = gm.pitch;
   const int blk = (int)blockIdx.x;
@@ -384,17 +152,17 @@ __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
   const bool col_in = (i > 1 && i < nx);
 
   for (int jj = j0 - 1; jj <= j0 + 1; ++jj) {
-    put(L, slot(jj), t + 1, load_node<EQF>(in, pa, gm, i, clampj(jj)));
-    if (t < 2) put(L, slot(jj), hslot, load_node<EQF>(in, pa, gm, hcol, clampj(jj)));
+    put(L, slot(jj), t + 1, load_node<R, EQF>(in, pa, gm, i, clampj(jj)));
+    if (t < 2) put(L, slot(jj), hslot, load_node<R, EQF>(in, pa, gm, hcol, clampj(jj)));
   }
   __syncthreads();
 
   for (int j = j0; j <= j1; ++j) {
     const bool pre = (j + 2 <= j1 + 1);
-    NodeM<EQF> nn, nh;
+    Node<R, EQF> nn, nh;
     if (pre) {
-      nn = load_node<EQF>(in, pa, gm, i, clampj(j + 2));
-      if (t < 2) nh = load_node<EQF>(in, pa, gm, hcol, clampj(j + 2));
+      nn = load_node<R, EQF>(in, pa, gm, i, clampj(j + 2));
+      if (t < 2) nh = load_node<R, EQF>(in, pa, gm, hcol, clampj(j + 2));
     }
     if (col_in && j > 1 && j < ny) {
       const int rS = slot(j - 1), rC = slot(j), rN = slot(j + 1);
@@ -408,13 +176,13 @@ __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
         const int r = CY[k] > 0 ? rS : (CY[k] < 0 ? rN : rC);
         const int ss = s - CX[k];
         double fk, gk;
-        rebuild(k, L.a[r][ss], L.b[r][ss], L.c[r][ss], &fk, &gk);
+        rebuild<R, EQF>(k, L.a[r][ss], L.b[r][ss], L.c[r][ss], &fk, &gk);
         pull_add(S, k, fk, gk);
       }
       const double tap[9] = {L.phi[rC][s],     L.phi[rC][s + 1], L.phi[rN][s],
                              L.phi[rC][s - 1], L.phi[rS][s],     L.phi[rN][s + 1],
                              L.phi[rN][s - 1], L.phi[rS][s - 1], L.phi[rS][s + 1]};
-      const NodeOut o = update_node<EQF>(c, S, tap, out, pb, q);
+      const NodeOut o = update_node<R, EQF>(c, S, tap, out, pb, q, probe_idx, probe_out);
       if (write_all) {
         m.rho[q] = o.rho;
         m.T[q] = o.Tav;
@@ -454,21 +222,23 @@ __device__ __forceinline__ bool edge_node(int e, int nx, int ny, int* i, int* j)
 // frozen corner ghosts, W/E thermal ghosts at Tin and at the previous
 // step's averaged T(Nx, j), N/S ghost rows of g and phi equal to rows Ny / 1;
 // then the halo bookkeeping of the macroscopic fields.
-template <bool EQF>
+template <typename R, bool EQF>
 __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& out,
                                                const double* __restrict__ pa,
                                                double* __restrict__ pb,
                                                const double* __restrict__ te_in,
                                                double* __restrict__ te_out, const Macro& m,
-                                               const Consts& c, const Geom& gm, int e) {
+                                               const Consts& c, const Geom& gm, int e,
+                                               const int* __restrict__ probe_idx,
+                                               double* __restrict__ probe_out) {
   const int nx = gm.nx, ny = gm.ny;
   const int64_t P = gm.pitch;
   int i, j;
   if (!edge_node(e, nx, ny, &i, &j)) return;
   const int64_t q = (int64_t)j * P + i;
   const double2 oa = in.a[q];
-  const float4 ob = in.b[q];
-  const typename CPlane<EQF>::T oc = c_at<EQF>(in.c, q);
+  const typename Vec<R>::V4 ob = b_at<R, EQF>(in.b, q);
+  const typename CType<R, EQF>::T oc = c_at<R, EQF>(in.c, q);
   double f[9], gd[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
@@ -478,7 +248,7 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
       // interior source column: rebuild from the node (si, sjc); for a ghost
       // row this is the N/S copy of g, and f is replaced below
       const int64_t h = (int64_t)sjc * P + si;
-      rebuild(k, in.a[h], in.b[h], c_at<EQF>(in.c, h), &f[k], &gd[k]);
+      rebuild<R, EQF>(k, in.a[h], b_at<R, EQF>(in.b, h), c_at<R, EQF>(in.c, h), &f[k], &gd[k]);
     } else {
       const int64_t h = (int64_t)sjc * P + si;
       gd[k] = ghost_gdev(k, si == 0 ? c.tin : te_in[sjc], m.u[h], m.v[h]);
@@ -489,7 +259,7 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
       else if (k == 7 && si == nx + 1 && sj == ny + 1)
         f[k] = c.frozen7;
       else
-        f[k] = rebuild_f(OPP[k], oa, ob, oc);
+        f[k] = rebuild_f<R, EQF>(OPP[k], oa, ob, oc);
     }
   }
   const int64_t rn = (int64_t)(j < ny ? j + 1 : ny) * P;
@@ -501,7 +271,7 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
   pull_zero(S);
 #pragma unroll
   for (int k = 0; k < 9; ++k) pull_add(S, k, f[k], gd[k]);
-  const NodeOut o = update_node<EQF>(c, S, tap, out, pb, q);
+  const NodeOut o = update_node<R, EQF>(c, S, tap, out, pb, q, probe_idx, probe_out);
 
   // ---- halo bookkeeping: BC_f N/S rows (rho copied, u = v = 0), BC_g
   // T and phi ghosts, the east column's averaged T for the next step
@@ -555,22 +325,23 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
   m.v[q] = o.vav;
 }
 
-template <int W, int NS, bool EQF>
+template <int W, int NS, typename R, bool EQF>
 __global__ __launch_bounds__(W) void k_fused_mom(
     MomIn in, MomOut out, const double* __restrict__ pa, double* __restrict__ pb,
     const double* __restrict__ te_in, double* __restrict__ te_out, Macro m, Consts c, Geom gm,
-    StripGrid sg, int write_all) {
+    StripGrid sg, int write_all, const int* __restrict__ probe_idx,
+    double* __restrict__ probe_out) {
   if ((int)blockIdx.x >= sg.nmain) {
     const int e = ((int)blockIdx.x - sg.nmain) * W + (int)threadIdx.x;
-    mom_edge_nodes<EQF>(in, out, pa, pb, te_in, te_out, m, c, gm, e);
+    mom_edge_nodes<R, EQF>(in, out, pa, pb, te_in, te_out, m, c, gm, e, probe_idx, probe_out);
     return;
   }
-  mom_interior<W, NS, EQF>(in, out, pa, pb, m, c, gm, sg, write_all);
+  mom_interior<W, NS, R, EQF>(in, out, pa, pb, m, c, gm, sg, write_all, probe_idx, probe_out);
 }
 
 // STEP_7/initialization.f90:15-152 for a moment handle: the initial
 // populations are equilibria, i.e. moments with zero non-equilibrium part.
-template <bool EQF>
+template <typename R, bool EQF>
 __global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* phip, Consts c,
                                      Geom gm, int interf, int west_bc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -579,8 +350,8 @@ __global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* ph
   const int64_t q = (int64_t)j * gm.pitch + i;
   const NodeInit s0 = initial_node(c, gm, i, j, interf, west_bc);
   st.a[q] = make_double2(s0.rho, s0.T - kTcrit);
-  st.b[q] = make_float4((float)s0.u, (float)s0.v, 0.f, 0.f);
-  static_cast<typename CPlane<EQF>::T*>(st.c)[q] = typename CPlane<EQF>::T{};
+  static_cast<typename Vec<R>::V4*>(st.b)[q] = make4<R>((R)s0.u, (R)s0.v, R(0), R(0));
+  static_cast<typename CType<R, EQF>::T*>(st.c)[q] = typename CType<R, EQF>::T{};
   m.rho[q] = s0.rho;
   m.u[q] = s0.u;
   m.v[q] = s0.v;
@@ -615,13 +386,12 @@ StripGrid make_strip_grid(const bgk_handle* h, int w, int target, int slots) {
   return s;
 }
 
-template <int W, int NS, bool EQF>
-void launch_mom_t(bgk_handle* h, MomIn in, MomOut out, const double* te_in, double* te_out,
-                  bool write_all) {
+template <int W, int NS, typename R, bool EQF>
+hipError_t launch_mom_t(bgk_handle* h, const MomStep& st) {
   // resident workgroups per CU of this instantiation (LDS and registers)
   static const int per_cu = [] {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_mom<W, NS, EQF>, W, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_mom<W, NS, R, EQF>, W, 0) !=
         hipSuccess)
       n = 0;
     return n;
@@ -631,47 +401,72 @@ void launch_mom_t(bgk_handle* h, MomIn in, MomOut out, const double* te_in, doub
   const int nedge = nx * (ny > 1 ? 2 : 1) + (ny > 2 ? (ny - 2) * (nx > 1 ? 2 : 1) : 0);
   const int64_t nwg = (int64_t)sg.nmain + (nedge + W - 1) / W;
   Macro m{h->rho, h->u, h->v, h->T};
-  hipLaunchKernelGGL((k_fused_mom<W, NS, EQF>), dim3((unsigned)nwg), dim3(W), 0, h->stream, in,
-                     out, h->phi[h->pc], h->phi[1 - h->pc], te_in, te_out, m, h->c, h->geom, sg,
-                     write_all ? 1 : 0);
+  if (st.macro[0]) m = Macro{st.macro[0], st.macro[1], st.macro[2], st.macro[3]};
+  hipLaunchKernelGGL((k_fused_mom<W, NS, R, EQF>), dim3((unsigned)nwg), dim3(W), 0, h->stream,
+                     st.in, st.out, st.pa, st.pb, st.te_in, st.te_out, m, h->c, h->geom, sg,
+                     st.write_all ? 1 : 0, st.probe_idx, st.probe_out);
+  return hipGetLastError();
 }
 
+// FP64 moments take at most 256 columns per workgroup (the 512-wide ring
+// of 88-byte nodes exceeds the 160 KiB of LDS).
 template <int W, int NS>
-void launch_mom(bgk_handle* h, MomIn in, MomOut out, const double* te_in, double* te_out,
-                bool write_all) {
+hipError_t launch_mom(bgk_handle* h, const MomStep& st) {
+  constexpr int W64 = W > 256 ? 256 : W;
+  const bool f64 = h->prm.precision == BGK_FP64_MOM;
   if (h->mom_eqf)
-    launch_mom_t<W, NS, true>(h, in, out, te_in, te_out, write_all);
-  else
-    launch_mom_t<W, NS, false>(h, in, out, te_in, te_out, write_all);
+    return f64 ? launch_mom_t<W64, NS, double, true>(h, st)
+               : launch_mom_t<W, NS, float, true>(h, st);
+  return f64 ? launch_mom_t<W64, NS, double, false>(h, st)
+             : launch_mom_t<W, NS, float, false>(h, st);
+}
+
+template <typename R, bool EQF>
+void launch_init(bgk_handle* h, const MomOut& st, dim3 grd) {
+  Macro m{h->rho, h->u, h->v, h->T};
+  hipLaunchKernelGGL((k_initialization_mom<R, EQF>), grd, dim3(TX, TY), 0, h->stream, st, m,
+                     h->phi[h->pc], h->phi[1 - h->pc], h->c, h->geom, h->geom.nx / 40,
+                     h->prm.west_bc);
 }
 
 }  // namespace
 
-hipError_t launch_fused_step_mom(bgk_handle* h, int step_parity, bool write_all) {
-  const double* te_in = h->teast[step_parity & 1];
-  double* te_out = h->teast[(step_parity + 1) & 1];
-  const int a = h->cur, b = 1 - h->cur;
-  MomIn in{h->ma[a], h->mb[a], h->mc[a]};
-  MomOut out{h->ma[b], h->mb[b], h->mc[b]};
-  // Tiling (workgroup width W, ring rows NS).  Measured at 4096^2 on MI355X:
-  // omega_f = 1 (36-byte nodes, 64-row strips) W = 512 / NS = 4 57.2 GLUPS,
-  // 512/3 53.1, 256/4 53.9 (32-row strips), 128/4 45.1, 64/4 45.8; general
-  // layout (48 bytes, 32-row strips) 512/3 45.9, 512/4 45.6, 256/3 42.0.
-  // Wider segments stream longer contiguous runs of each row.
+hipError_t launch_mom_step(bgk_handle* h, const MomStep& st) {
+  // Tiling (workgroup width W, ring rows NS).  Measured at 4096^2 on MI355X
+  // with FP32 moments: omega_f = 1 (36-byte nodes, 64-row strips) W = 512 /
+  // NS = 4 57.2 GLUPS, 512/3 53.1, 256/4 53.9 (32-row strips), 128/4 45.1,
+  // 64/4 45.8; general layout (48 bytes, 32-row strips) 512/3 45.9, 512/4
+  // 45.6, 256/3 42.0.  Wider segments stream longer contiguous runs of each
+  // row.  FP64 moments take the same tiling with half the workgroups per CU.
   int v = h->mom_variant;
   if (v == 0)
     v = h->nx >= 2048 ? (h->mom_eqf ? 7 : 8) : h->nx >= 512 ? 5 : h->nx >= 128 ? 3 : 1;
   switch (v) {
-    case 1: launch_mom<64, 4>(h, in, out, te_in, te_out, write_all); break;
-    case 2: launch_mom<64, 3>(h, in, out, te_in, te_out, write_all); break;
-    case 3: launch_mom<128, 4>(h, in, out, te_in, te_out, write_all); break;
-    case 4: launch_mom<128, 3>(h, in, out, te_in, te_out, write_all); break;
-    case 5: launch_mom<256, 4>(h, in, out, te_in, te_out, write_all); break;
-    case 6: launch_mom<256, 3>(h, in, out, te_in, te_out, write_all); break;
-    case 7: launch_mom<512, 4>(h, in, out, te_in, te_out, write_all); break;
-    default: launch_mom<512, 3>(h, in, out, te_in, te_out, write_all); break;
+    case 1: return launch_mom<64, 4>(h, st);
+    case 2: return launch_mom<64, 3>(h, st);
+    case 3: return launch_mom<128, 4>(h, st);
+    case 4: return launch_mom<128, 3>(h, st);
+    case 5: return launch_mom<256, 4>(h, st);
+    case 6: return launch_mom<256, 3>(h, st);
+    case 7: return launch_mom<512, 4>(h, st);
+    default: return launch_mom<512, 3>(h, st);
   }
-  hipError_t e = hipGetLastError();
+}
+
+hipError_t launch_fused_step_mom(bgk_handle* h, int step_parity, bool write_all) {
+  const int a = h->cur, b = 1 - h->cur;
+  MomStep st;
+  st.in = MomIn{h->ma[a], h->mb[a], h->mc[a]};
+  st.out = MomOut{h->ma[b], h->mb[b], h->mc[b]};
+  st.pa = h->phi[h->pc];
+  st.pb = h->phi[1 - h->pc];
+  st.te_in = h->teast[step_parity & 1];
+  st.te_out = h->teast[(step_parity + 1) & 1];
+  st.write_all = write_all;
+  st.probe_idx = nullptr;
+  st.probe_out = nullptr;
+  for (int k = 0; k < 4; ++k) st.macro[k] = nullptr;
+  hipError_t e = launch_mom_step(h, st);
   if (e == hipSuccess) {
     h->cur = b;
     h->pc = 1 - h->pc;
@@ -680,16 +475,14 @@ hipError_t launch_fused_step_mom(bgk_handle* h, int step_parity, bool write_all)
 }
 
 hipError_t launch_initialization_mom(bgk_handle* h) {
-  Macro m{h->rho, h->u, h->v, h->T};
   const Geom& g = h->geom;
   MomOut st{h->ma[h->cur], h->mb[h->cur], h->mc[h->cur]};
   dim3 grd((unsigned)((g.nx + 2 + TX - 1) / TX), (unsigned)((g.ny + 2 + TY - 1) / TY));
+  const bool f64 = h->prm.precision == BGK_FP64_MOM;
   if (h->mom_eqf)
-    hipLaunchKernelGGL(k_initialization_mom<true>, grd, dim3(TX, TY), 0, h->stream, st, m,
-                       h->phi[h->pc], h->phi[1 - h->pc], h->c, g, g.nx / 40, h->prm.west_bc);
+    f64 ? launch_init<double, true>(h, st, grd) : launch_init<float, true>(h, st, grd);
   else
-    hipLaunchKernelGGL(k_initialization_mom<false>, grd, dim3(TX, TY), 0, h->stream, st, m,
-                       h->phi[h->pc], h->phi[1 - h->pc], h->c, g, g.nx / 40, h->prm.west_bc);
+    f64 ? launch_init<double, false>(h, st, grd) : launch_init<float, false>(h, st, grd);
   return hipGetLastError();
 }
 

@@ -8,6 +8,10 @@
 #include "../../include/bgk_pcm.h"
 #include "bgk_device.h"
 
+namespace bgk {
+struct AdjCtx;  // adjoint context (bgk_adjoint.hip)
+}
+
 // One lattice resident in HBM.
 //
 // Layout: every field is a plane of (Ny+2) rows of `pitch` nodes; node
@@ -28,6 +32,7 @@
 // (with omega_f = 1: mb = (u, v, Pg_xx, Pg_yy), mc = Pg_xy).
 struct bgk_handle {
   bgk_params prm;
+  bgk_ext ext;  // physics extensions (defaults: STEP_7)
   int nx, ny;
   int64_t pitch;        // elements per row
   int64_t plane_elems;  // nodes per plane: (ny+2)*pitch
@@ -47,8 +52,8 @@ struct bgk_handle {
   float* g32[2][9];
   // FP32 moment state (moment handles)
   double2* ma[2];
-  float4* mb[2];
-  void* mc[2];    // float4 planes, or float planes when mom_eqf
+  void* mb[2];    // float4 (FP32 moments) or double4 (FP64 moments) planes
+  void* mc[2];    // the same, or float / double planes when mom_eqf
   bool mom_eqf;   // omega_f == 1: Pf vanishes, mb = (u, v, Pg_xx, Pg_yy), mc = Pg_xy
   int strip_rows;   // rows per workgroup strip of the moment kernel (0: default)
   int mom_variant;  // tiling variant of the moment kernel (0: default)
@@ -66,6 +71,7 @@ struct bgk_handle {
   bgk::Consts c;
   bgk::Geom geom;
   bool fused_ok;  // all four walls no-slip (BC type 4)
+  bgk::AdjCtx* adj;  // adjoint context, null until bgk_adj_setup
 };
 
 namespace bgk {
@@ -109,6 +115,13 @@ hipError_t launch_gather(bgk_handle* h, const double* plane, int n, const int32_
                          double* out_dev);
 
 int reduce_partial_count(const bgk_handle* h);
+
+// ---- adjoint (bgk_adjoint.hip), FP64 moment handles
+hipError_t adj_setup(bgk_handle* h, int nsteps, int seg, int np, const int32_t* pi,
+                     const int32_t* pj);
+hipError_t adj_forward(bgk_handle* h, double* probe_T_host);
+hipError_t adj_backward(bgk_handle* h, const double* seeds_host, double grad[3]);
+void adj_free(bgk_handle* h);
 
 }  // namespace bgk
 

@@ -56,6 +56,11 @@ class BgkInfoC(ctypes.Structure):
     ]
 
 
+class BgkExtC(ctypes.Structure):
+    """struct bgk_ext of include/bgk_pcm.h (physics extensions)."""
+    _fields_ = [("t_hot", ctypes.c_double), ("k_ratio", ctypes.c_double)]
+
+
 _lib = None
 
 # name -> (argtypes) for every entry point of include/bgk_pcm.h
@@ -69,6 +74,7 @@ SIGNATURES = {
     "bgk_destroy": [_H],
     "bgk_set_stream": [_H, ctypes.c_void_p],
     "bgk_initialization": [_H],
+    "bgk_set_extensions": [_H, ctypes.POINTER(BgkExtC)],
     "bgk_upload": [_H, ctypes.c_int32, _D, ctypes.c_int64],
     "bgk_download": [_H, ctypes.c_int32, _D, ctypes.c_int64],
     "bgk_bc_f": [_H],
@@ -86,6 +92,9 @@ SIGNATURES = {
     "bgk_gather": [_H, ctypes.c_int32, ctypes.c_int32, _I32P, _I32P, _D],
     "bgk_synchronize": [_H],
     "bgk_get_info": [_H, ctypes.POINTER(BgkInfoC)],
+    "bgk_adj_setup": [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _I32P, _I32P],
+    "bgk_adj_forward": [_H, _D],
+    "bgk_adj_backward": [_H, _D, _D],
 }
 
 
@@ -114,9 +123,14 @@ def _check(rc: int) -> None:
 
 
 class Lattice:
-    """One lattice on the GPU; see the module docstring."""
+    """One lattice on the GPU; see the module docstring.
 
-    def __init__(self, params: Params, initialize: bool = True):
+    t_hot, k_ratio: physics extensions of bgk_set_extensions (hot-wall and
+    initial liquid temperature; solid/liquid conductivity ratio), applied
+    before the initialization; the defaults give STEP_7."""
+
+    def __init__(self, params: Params, initialize: bool = True, t_hot: float = 0.0,
+                 k_ratio: float = 1.0):
         self.params = params
         self.nx, self.ny = params.nx, params.ny
         self._lib = load_library()
@@ -125,8 +139,38 @@ class Lattice:
         _check(self._lib.bgk_create(ctypes.byref(self._c), ctypes.byref(h)))
         self._h = h
         self.it = 0
+        if t_hot != 0.0 or k_ratio != 1.0:
+            self.set_extensions(t_hot, k_ratio)
         if initialize:
             self.initialization()
+
+    def set_extensions(self, t_hot: float = 0.0, k_ratio: float = 1.0) -> None:
+        e = BgkExtC(t_hot, k_ratio)
+        _check(self._lib.bgk_set_extensions(self._h, ctypes.byref(e)))
+
+    # ---- adjoint (FP64 moment handles): probe-temperature record and its
+    # gradient with respect to (stefan_number, k_ratio, t_hot)
+    def adj_setup(self, nsteps: int, seg_len: int, pi: Sequence[int], pj: Sequence[int]) -> None:
+        self._adj_n, self._adj_np = int(nsteps), len(pi)
+        ai = np.ascontiguousarray(pi, dtype=np.int32)
+        aj = np.ascontiguousarray(pj, dtype=np.int32)
+        _check(self._lib.bgk_adj_setup(self._h, nsteps, seg_len, len(pi),
+                                       ai.ctypes.data_as(_I32P), aj.ctypes.data_as(_I32P)))
+
+    def adj_forward(self) -> np.ndarray:
+        """Run the recorded forward; returns T at the probes, (nsteps, n_probes)."""
+        out = np.zeros((self._adj_n, max(self._adj_np, 1)))
+        _check(self._lib.bgk_adj_forward(self._h, out.ctypes.data_as(_D)))
+        self.it += self._adj_n
+        return out[:, :self._adj_np].copy()
+
+    def adj_backward(self, seeds: np.ndarray) -> np.ndarray:
+        """dL/d(stefan_number, k_ratio, t_hot) from seeds = dL/dT(step, probe)."""
+        sd = np.zeros((self._adj_n, max(self._adj_np, 1)))
+        sd[:, :self._adj_np] = np.asarray(seeds, dtype=np.float64).reshape(self._adj_n, -1)
+        g = np.zeros(3)
+        _check(self._lib.bgk_adj_backward(self._h, sd.ctypes.data_as(_D), g.ctypes.data_as(_D)))
+        return g
 
     # ---- lifetime
     def close(self) -> None:

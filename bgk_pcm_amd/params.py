@@ -21,6 +21,7 @@ from typing import Dict
 FP64 = 0
 FP32_DEV = 1
 FP32_MOM = 2
+FP64_MOM = 3
 ROUND_STEP7 = 0
 ROUND_CLEAN = 1
 
@@ -146,8 +147,8 @@ class Params:
                 raise ValueError(f"{k}={getattr(self, k)} is not one of {allowed}")
         if self.tau_f == 0.0 or self.tau_g == 0.0:
             raise ValueError("tau_f and tau_g must be nonzero")
-        if self.precision not in (FP64, FP32_DEV, FP32_MOM):
-            raise ValueError("precision must be FP64, FP32_DEV or FP32_MOM")
+        if self.precision not in (FP64, FP32_DEV, FP32_MOM, FP64_MOM):
+            raise ValueError("precision must be FP64, FP32_DEV, FP32_MOM or FP64_MOM")
         if self.rounding not in (ROUND_STEP7, ROUND_CLEAN):
             raise ValueError("rounding must be ROUND_STEP7 or ROUND_CLEAN")
 

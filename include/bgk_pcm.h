@@ -49,7 +49,7 @@ enum {
   BGK_FP64 = 0,     /* f, g stored and computed in FP64                        */
   BGK_FP32_DEV = 1, /* f - w*rho0 and g - w*T_crit (T_crit = 273) stored in
                        FP32; phase field, enthalpy update and moments in FP64  */
-  BGK_FP32_MOM = 2  /* moment storage: per node rho and T - T_crit in FP64,
+  BGK_FP32_MOM = 2, /* moment storage: per node rho and T - T_crit in FP64,
                        u, v and the non-equilibrium second moments of f and g
                        (times 1 - omega) in FP32, 48 bytes instead of 144;
                        populations are rebuilt from them at each pull.  Clean
@@ -57,6 +57,8 @@ enum {
                        ft, gt are not representable); no-slip cavity only;
                        populations are not addressable (download/upload of
                        f, g, fp, gp, rho2..T2 and all uploads: unsupported) */
+  BGK_FP64_MOM = 3  /* the same moment storage with every moment in FP64: the
+                       forward model of the adjoint (bgk_adj_*)              */
 };
 
 /* rounding policy of the FP64 arithmetic (bgk_params.rounding) */
@@ -115,7 +117,7 @@ typedef struct bgk_params {
   int32_t east_bc;       /* id(19) 0 periodic,3 pressure,4 no-slip          */
   int32_t north_bc;      /* id(20) 0 periodic,1 velocity,4 no-slip,5 symmetric,6 open */
   int32_t south_bc;      /* id(21) 0 periodic,4 no-slip,6 open              */
-  int32_t precision;     /* BGK_FP64, BGK_FP32_DEV or BGK_FP32_MOM          */
+  int32_t precision;     /* BGK_FP64, BGK_FP32_DEV, BGK_FP32_MOM, BGK_FP64_MOM */
   int32_t rounding;      /* BGK_ROUND_STEP7 or BGK_ROUND_CLEAN              */
   int32_t device;        /* HIP device ordinal                              */
   int32_t fp32_collision;/* FP32 handles: 1 = collision arithmetic in FP32,
@@ -131,6 +133,18 @@ typedef struct bgk_info {
   int32_t pitch;            /* device row pitch in elements               */
   int32_t fused_path;       /* 1 if bgk_step runs the fused kernel        */
 } bgk_info;
+
+/* Physics extensions (not in the reference; the defaults t_hot = 0 ->
+ * T0 and k_ratio = 1 give STEP_7's step).  Honoured by moment handles
+ * (BGK_FP32_MOM, BGK_FP64_MOM); other handles accept only the defaults. */
+typedef struct bgk_ext {
+  double t_hot;    /* hot (west) wall and initial liquid temperature in place
+                      of T0 (0: T0); the scale T0 - T_in_sol of
+                      STEP_7/forces.f90:12-13 is unchanged                  */
+  double k_ratio;  /* solid/liquid thermal conductivity ratio: alpha(phi) =
+                      alpha_l (fl + k_ratio (1 - fl)), fl = (1 + phi)/2,
+                      tau_g = 1/2 + alpha/cs2 (alpha_l from tau_g)          */
+} bgk_ext;
 
 typedef struct bgk_handle bgk_handle;
 
@@ -150,6 +164,12 @@ int bgk_set_stream(bgk_handle* h, void* hip_stream);
 /* STEP_7/initialization.f90:5-152: initial rho, u, v, T, phi, phi_prec and
  * equilibrium populations on every node, halo included. */
 int bgk_initialization(bgk_handle* h);
+
+/* Set the physics extensions (bgk_ext); takes effect from the next
+ * bgk_initialization.  No reference counterpart: the extensions express
+ * the configurations STEP_7 cannot (hot-wall temperature as a parameter of
+ * its own, conductivity ratio). */
+int bgk_set_extensions(bgk_handle* h, const bgk_ext* e);
 
 /* Copy one field between host and device.  src/dst hold (Nx+2) x (Ny+2)
  * doubles with leading dimension ld >= Nx+2.  Replaces the host<->device
@@ -202,6 +222,26 @@ int bgk_gather(bgk_handle* h, int32_t field, int32_t n, const int32_t* i,
 
 /* Wait for all queued work of the handle. */
 int bgk_synchronize(bgk_handle* h);
+
+/* ---- Adjoint of the time step (no reference counterpart: the reference
+ * has no derivatives).  Differentiates the FP64 moment step (precision
+ * BGK_FP64_MOM, no-slip cavity) for a loss that depends on the averaged
+ * temperature T (STEP_7/moments.f90:11-25) at n_probes interior nodes
+ * (pi[p], pj[p]) after each of nsteps steps.  The state is checkpointed
+ * every seg_len steps and recomputed segment by segment.
+ *
+ * bgk_adj_setup     allocates the checkpoints and adjoint state.
+ * bgk_adj_forward   runs nsteps steps from the current state (the
+ *                   initialization, for the t_hot gradient to include the
+ *                   initial liquid temperature), writing the probe record
+ *                   probe_T[n*n_probes + p] = T after step n+1.
+ * bgk_adj_backward  takes seeds[n*n_probes + p] = dL/dprobe_T[...] and
+ *                   returns grad = dL/d(stefan_number, k_ratio, t_hot)
+ *                   (k_ratio and t_hot: bgk_ext). */
+int bgk_adj_setup(bgk_handle* h, int32_t nsteps, int32_t seg_len, int32_t n_probes,
+                  const int32_t* pi, const int32_t* pj);
+int bgk_adj_forward(bgk_handle* h, double* probe_T);
+int bgk_adj_backward(bgk_handle* h, const double* seeds, double grad[3]);
 
 /* Device memory held, device totals and host RSS. */
 int bgk_get_info(bgk_handle* h, bgk_info* info);

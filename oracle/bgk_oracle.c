@@ -33,6 +33,8 @@ struct oracle {
   double cs2, omega_f, omega_g, kvisc, alpha;
   double u0, v0, tin;
   int interf;
+  /* extensions (off: t_hot = T0, k_ratio = 1) */
+  double t_hot, k_ratio;
   double* fld[BGK_NFIELDS];
 };
 
@@ -74,10 +76,24 @@ oracle_t* oracle_create(const bgk_params* p) {
    * the parameters only, so a state loaded without initialization (a
    * snapshot) gets them too. */
   s->interf = p->nx / 40;
-  s->tin = p->t0;
+  s->t_hot = p->t0;
+  s->k_ratio = 1.0;
+  s->tin = s->t_hot;
   s->u0 = p->re * s->kvisc / p->length;
   s->v0 = 0.0;
   return s;
+}
+
+/* Physics extensions (not in the reference; off = STEP_7): the hot-wall
+ * and initial liquid temperature t_hot in place of T0 (the temperature
+ * scale T0 - T_in_sol of forces.f90:12-13 is kept), and a solid/liquid
+ * conductivity ratio through a phase-dependent thermal relaxation time,
+ * alpha(phi) = alpha_l (fl + k_ratio (1 - fl)), fl = (1 + phi)/2,
+ * tau_g = 1/2 + alpha/cs2.  Call before oracle_initialization. */
+void oracle_set_ext(oracle_t* s, double t_hot, double k_ratio) {
+  s->t_hot = t_hot;
+  s->k_ratio = k_ratio;
+  s->tin = t_hot;
 }
 
 void oracle_destroy(oracle_t* s) {
@@ -120,12 +136,12 @@ void oracle_initialization(oracle_t* s) {
   double *phi = F[BGK_PHI], *phip = F[BGK_PHI_PREC];
 
   s->interf = nx / 40;                    /* initialization.f90:5   */
-  s->tin = p->t0;                         /* :9                     */
+  s->tin = s->t_hot;                      /* :9 (T0 unless extended) */
   s->u0 = p->re * s->kvisc / p->length;   /* :11                    */
   s->v0 = 0.0;                            /* :12                    */
   for (int j = 0; j <= ny + 1; ++j) {     /* :15-26                 */
     for (int i = 0; i <= s->interf && i <= nx + 1; ++i) {
-      T[IX(i, j)] = p->t0;
+      T[IX(i, j)] = s->t_hot;
       rho[IX(i, j)] = p->rho1;
     }
     for (int i = s->interf + 1; i <= nx + 1; ++i) {
@@ -512,7 +528,13 @@ void oracle_collision(oracle_t* s) { /* collision.f90:14-152 */
       regularised_bgk(s->w, s->cs2, s->omega_f, s->step7, rho[q], u[q], v[q], pop, out);
       for (int k = 0; k < 9; ++k) F[BGK_F0 + k][q] = out[k];
       for (int k = 0; k < 9; ++k) pop[k] = F[BGK_GP0 + k][q];
-      regularised_bgk(s->w, s->cs2, s->omega_g, s->step7, T[q], u[q], v[q], pop, out);
+      double omg = s->omega_g;
+      if (s->k_ratio != 1.0) { /* extension: phase-dependent conductivity */
+        double fl = 0.5 * (1.0 + F[BGK_PHI][q]);
+        double alpha = s->alpha * (fl + s->k_ratio * (1.0 - fl));
+        omg = 1.0 / (0.5 + alpha / s->cs2);
+      }
+      regularised_bgk(s->w, s->cs2, omg, s->step7, T[q], u[q], v[q], pop, out);
       for (int k = 0; k < 9; ++k) F[BGK_G0 + k][q] = out[k];
     }
 }

@@ -36,6 +36,10 @@ typedef struct oracle oracle_t;
 /* STEP_7/setup.f90:34-115 (constants and allocation). */
 oracle_t* oracle_create(const bgk_params* p);
 void oracle_destroy(oracle_t* s);
+/* Physics extensions of this library (not in the reference; t_hot = T0 and
+ * k_ratio = 1 give STEP_7): hot-wall / initial liquid temperature and the
+ * solid/liquid conductivity ratio.  Call before oracle_initialization. */
+void oracle_set_ext(oracle_t* s, double t_hot, double k_ratio);
 
 /* STEP_7/initialization.f90:5-152 */
 void oracle_initialization(oracle_t* s);

@@ -58,6 +58,7 @@ def lib() -> ctypes.CDLL:
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_diag.argtypes = [ctypes.c_void_p, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+        L.oracle_set_ext.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.oracle_tanh.restype = ctypes.c_double
         L.oracle_tanh.argtypes = [ctypes.c_double]
         _lib = L
@@ -67,13 +68,16 @@ def lib() -> ctypes.CDLL:
 class Oracle:
     """One CPU lattice of the reference STEP_7 (see oracle/bgk_oracle.h)."""
 
-    def __init__(self, p: Params, initialize: bool = True):
+    def __init__(self, p: Params, initialize: bool = True, t_hot: float | None = None,
+                 k_ratio: float = 1.0):
         self.p = p
         self.nx, self.ny = p.nx, p.ny
         self._c = p.to_c()
         self.h = lib().oracle_create(ctypes.byref(self._c))
         if not self.h:
             raise RuntimeError("oracle_create failed")
+        if t_hot is not None or k_ratio != 1.0:
+            lib().oracle_set_ext(self.h, p.t0 if t_hot is None else t_hot, k_ratio)
         if initialize:
             lib().oracle_initialization(self.h)
 

@@ -68,7 +68,7 @@ def test_invalid_parameters_are_rejected_before_touching_the_device():
     lib = lattice.load_library()
     h = ctypes.c_void_p()
     for bad in (Params(nx=0, ny=8), Params(nx=8, ny=8, west_bc=7), Params(nx=8, ny=8, east_bc=1),
-                Params(nx=8, ny=8, tau_f=0.0), Params(nx=8, ny=8, precision=3)):
+                Params(nx=8, ny=8, tau_f=0.0), Params(nx=8, ny=8, precision=4)):
         c = bad.to_c()
         assert lib.bgk_create(ctypes.byref(c), ctypes.byref(h)) == -1
         assert lib.bgk_last_error()

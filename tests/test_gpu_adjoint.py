@@ -1,0 +1,116 @@
+"""The GPU adjoint (bgk_adj_*) against torch.autograd on the CPU reference
+(tests/torch_ref.py, pinned to the oracle) and against central finite
+differences of the GPU forward.
+
+Loss: mean squared error of T at seeded-random interior probes after every
+step against a target trajectory of the same model at other parameters,
+gradient with respect to theta = (stefan_number, k_ratio, t_hot).
+Tolerances: probe record vs the CPU reference 1e-11 relative (FP64 moment
+form vs population form: only the order of roundings differs); gradient
+vs autograd 1e-7 relative per component; gradient vs central differences
+(steps 1e-6 St, 1e-6 k_ratio, 1e-4 t_hot) 1e-4 relative, the north star's
+bar.
+"""
+import numpy as np
+import pytest
+import torch
+
+from bgk_pcm_amd.adjoint import probe_temperature, seeded_probes
+from bgk_pcm_amd.lattice import BgkError, Lattice
+from bgk_pcm_amd.params import FP32_MOM, FP64_MOM, ROUND_CLEAN, Params
+from torch_ref import probe_record
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_record(p, theta, pi, pj, nsteps, seg):
+    st, kr, th = theta
+    with Lattice(p.replace(precision=FP64_MOM, rounding=ROUND_CLEAN, stefan_number=st),
+                 initialize=False, t_hot=th, k_ratio=kr) as lat:
+        lat.initialization()
+        lat.adj_setup(nsteps, seg, pi, pj)
+        return lat.adj_forward()
+
+
+@pytest.mark.parametrize("tau_f,seg", [(1.0, 7), (0.8, 50), (1.0, 1)])
+def test_adjoint_equals_autograd_of_the_cpu_reference(tau_f, seg):
+    p = Params(nx=30, ny=24, length=30.0, grav=2e-3, rounding=ROUND_CLEAN, tau_f=tau_f,
+               stefan_number=0.1)
+    nsteps = 120
+    pi, pj = seeded_probes(p.nx, p.ny, 6, seed=4)
+    # probes on the boundary-adjacent rows and columns too
+    pi += [1, p.nx, 5]
+    pj += [3, p.ny, 1]
+    theta0 = (0.1, 1.5, 283.5)
+    target = probe_record(p, torch.tensor([0.12, 1.2, 283.2], dtype=torch.float64), pi, pj,
+                          nsteps).numpy()
+    print(f"adjoint vs autograd: {p.nx}x{p.ny}, {nsteps} steps, seg_len {seg}, tau_f {tau_f}",
+          flush=True)
+    # CPU reference gradient
+    th = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    rec_ref = probe_record(p, th, pi, pj, nsteps)
+    loss = ((rec_ref - torch.from_numpy(target)) ** 2).mean()
+    loss.backward()
+    g_ref = th.grad.numpy()
+    # GPU record and adjoint gradient
+    with Lattice(p.replace(precision=FP64_MOM, stefan_number=theta0[0]), initialize=False,
+                 t_hot=theta0[2], k_ratio=theta0[1]) as lat:
+        lat.initialization()
+        lat.adj_setup(nsteps, seg, pi, pj)
+        rec = lat.adj_forward()
+        seeds = 2.0 * (rec - target) / rec.size
+        g = lat.adj_backward(seeds)
+    r = rec_ref.detach().numpy()
+    assert np.abs(rec - r).max() <= 1e-11 * np.abs(r).max()
+    print("gradient GPU", g, "autograd", g_ref)
+    np.testing.assert_allclose(g, g_ref, rtol=1e-7, atol=0)
+
+
+def test_adjoint_matches_central_differences():
+    p = Params(nx=64, ny=64, length=64.0, grav=1e-3, rounding=ROUND_CLEAN, stefan_number=0.1)
+    nsteps, seg = 400, 40
+    pi, pj = seeded_probes(p.nx, p.ny, 16, seed=1)
+    theta0 = np.array([0.1, 1.5, 283.5])
+    target = gpu_record(p, (0.12, 1.2, 283.2), pi, pj, nsteps, seg)
+    loss = lambda rec: float(((rec - target) ** 2).mean())
+    with Lattice(p.replace(precision=FP64_MOM, stefan_number=theta0[0]), initialize=False,
+                 t_hot=theta0[2], k_ratio=theta0[1]) as lat:
+        lat.initialization()
+        lat.adj_setup(nsteps, seg, pi, pj)
+        rec = lat.adj_forward()
+        g = lat.adj_backward(2.0 * (rec - target) / rec.size)
+    print("adjoint gradient", g, flush=True)
+    for c, h in enumerate((1e-6, 1e-6, 1e-4)):
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[c] += h
+        tm[c] -= h
+        fd = (loss(gpu_record(p, tp, pi, pj, nsteps, seg)) -
+              loss(gpu_record(p, tm, pi, pj, nsteps, seg))) / (2 * h)
+        print(f"component {c}: adjoint {g[c]:.10e} central difference {fd:.10e}")
+        assert abs(fd - g[c]) <= 1e-4 * abs(g[c]), (c, fd, g[c])
+
+
+def test_autograd_function_gives_the_adjoint_gradient():
+    p = Params(nx=24, ny=20, length=24.0, grav=2e-3, rounding=ROUND_CLEAN)
+    pi, pj = seeded_probes(p.nx, p.ny, 4, seed=9)
+    nsteps = 60
+    target = probe_record(p, torch.tensor([0.12, 1.2, 283.2], dtype=torch.float64), pi, pj,
+                          nsteps)
+    theta = torch.tensor([0.1, 1.5, 283.5], dtype=torch.float64, requires_grad=True)
+    T = probe_temperature(theta, p, pi, pj, nsteps, seg_len=16)
+    loss = ((T - target) ** 2).mean()
+    loss.backward()
+    th = torch.tensor([0.1, 1.5, 283.5], dtype=torch.float64, requires_grad=True)
+    ((probe_record(p, th, pi, pj, nsteps) - target) ** 2).mean().backward()
+    np.testing.assert_allclose(theta.grad.numpy(), th.grad.numpy(), rtol=1e-7)
+
+
+def test_adjoint_needs_an_fp64_moment_handle():
+    with Lattice(Params(nx=8, ny=8, precision=FP32_MOM)) as lat:
+        with pytest.raises(BgkError) as e:
+            lat.adj_setup(10, 5, [2], [3])
+        assert e.value.code == -4
+    with Lattice(Params(nx=8, ny=8, precision=FP64_MOM)) as lat:
+        with pytest.raises(BgkError) as e:
+            lat.adj_setup(10, 5, [0], [3])
+        assert e.value.code == -1

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from bgk_pcm_amd.lattice import BgkError, Lattice
-from bgk_pcm_amd.params import FP32_MOM, ROUND_CLEAN, Params
+from bgk_pcm_amd.params import FP32_MOM, FP64_MOM, ROUND_CLEAN, Params
 from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -130,3 +130,49 @@ def test_mom_handle_rejects_population_access():
     with pytest.raises(BgkError) as e:
         Lattice(Params(nx=8, ny=8, precision=FP32_MOM, north_bc=0, south_bc=0))
     assert e.value.code == -4
+
+
+# ---- FP64 moment storage (the adjoint's forward model) and the extensions
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (3, 2), (37, 29), (130, 5), (300, 9)])
+@pytest.mark.parametrize("tau_f", [1.0, 0.8])
+def test_fp64_mom_matches_clean_oracle(nx, ny, tau_f):
+    """FP64 moments: the clean FP64 step in moment form; only the order of
+    the FP64 roundings differs from the population form of the oracle."""
+    p = Params(nx=nx, ny=ny, length=float(max(nx, ny)), grav=1e-3, rounding=ROUND_CLEAN,
+               tau_f=tau_f)
+    print(f"FP64 moments {nx}x{ny} tau_f={tau_f}", flush=True)
+    o = Oracle(p)
+    o.step(60)
+    with Lattice(p.replace(precision=FP64_MOM)) as lat:
+        lat.step(60)
+        err = rel_errors(lat, o, interior=False, speed_scale=True)
+    print("relative max-norm errors:", err)
+    assert max(err.values()) <= 1e-11, err
+
+
+@pytest.mark.parametrize("precision,tol", [(FP64_MOM, 1e-10), (FP32_MOM, 1e-6)])
+@pytest.mark.parametrize("t_hot,k_ratio", [(283.5, 1.0), (283.0, 1.5), (285.0, 0.5)])
+def test_extensions_match_oracle(precision, tol, t_hot, k_ratio):
+    """Hot-wall temperature and conductivity ratio against the oracle's
+    restatement of the same extension, 300 steps on 48 x 40."""
+    p = Params(nx=48, ny=40, length=48.0, grav=1e-3, rounding=ROUND_CLEAN)
+    o = Oracle(p, t_hot=t_hot, k_ratio=k_ratio)
+    o.step(300)
+    with Lattice(p.replace(precision=precision), t_hot=t_hot, k_ratio=k_ratio) as lat:
+        lat.step(300)
+        err = rel_errors(lat, o, interior=False, speed_scale=True)
+    print(f"extensions t_hot={t_hot} k_ratio={k_ratio}: {err}")
+    assert max(err.values()) <= tol, err
+
+
+def test_extensions_need_a_moment_handle():
+    with Lattice(Params(nx=8, ny=8)) as lat:
+        lat.set_extensions(0.0, 1.0)  # the defaults are accepted everywhere
+        with pytest.raises(BgkError) as e:
+            lat.set_extensions(284.0, 1.0)
+        assert e.value.code == -4
+    with Lattice(Params(nx=8, ny=8, precision=FP64_MOM)) as lat:
+        with pytest.raises(BgkError) as e:
+            lat.set_extensions(284.0, 0.0)
+        assert e.value.code == -1

@@ -192,3 +192,42 @@ def test_melting_is_monotone_without_convection():
         melted.append(np.count_nonzero(o.field("phi")[1:-1, 1:-1] > 0))
     assert all(b >= a for a, b in zip(melted, melted[1:]))
     assert melted[-1] > melted[0] > 0
+
+
+@pytest.mark.parametrize("k_ratio", [1.5, 0.4])
+def test_conductivity_ratio_scales_the_thermal_relaxation(k_ratio):
+    """Extension (no reference counterpart): the regularized collision
+    scales the non-equilibrium second moment of g by 1 - omega_g, and the
+    thermal diffusivity is alpha = cs2 (tau_g - 1/2).  With the conductivity
+    ratio, the solid (phi = -1) must relax with alpha_s = k_ratio alpha_l,
+    the liquid (phi = +1) with alpha_l, and phi = 0 with their mean."""
+    tau_g = 0.8
+    rng = np.random.default_rng(3)
+    o = Oracle(Params(nx=12, ny=9, tau_g=tau_g, rounding=1), initialize=False,
+               k_ratio=k_ratio)
+    shape = o.field("T").shape
+    phi = rng.choice([-1.0, 0.0, 1.0], size=shape)
+    T = rng.uniform(270.0, 285.0, shape)
+    u = rng.uniform(-1e-3, 1e-3, shape)
+    v = rng.uniform(-1e-3, 1e-3, shape)
+    o.field("phi")[...] = phi
+    o.field("T")[...] = T
+    o.field("u")[...] = u
+    o.field("v")[...] = v
+    o.field("rho")[...] = 1.0
+    uu = 1.5 * (u * u + v * v)
+    geq = [W[k] * T * (1 + 3 * (CX[k] * u + CY[k] * v) + 4.5 * (CX[k] * u + CY[k] * v) ** 2 - uu)
+           for k in range(9)]
+    for k in range(9):
+        o.field(f"gp{k}")[...] = geq[k] * (1 + rng.uniform(-1e-3, 1e-3, shape))
+        o.field(f"fp{k}")[...] = W[k]
+    o.stage("collision")
+    pxx = lambda pops: sum((CX[k] ** 2 - 1 / 3) * (pops[k] - geq[k]) for k in range(9))
+    before = pxx([o.field(f"gp{k}") for k in range(9)])
+    after = pxx([o.field(f"g{k}") for k in range(9)])
+    alpha_l = (tau_g - 0.5) / 3
+    alpha = np.where(phi < 0, k_ratio * alpha_l,
+                     np.where(phi > 0, alpha_l, 0.5 * (1 + k_ratio) * alpha_l))
+    expect = 1 - 1 / (0.5 + 3 * alpha)
+    inner = (slice(1, -1), slice(1, -1))
+    np.testing.assert_allclose((after / before)[inner], expect[inner], rtol=1e-9)
