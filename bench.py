@@ -23,6 +23,13 @@ Printed JSON line (one):
                  length) of both FP32 storages against the FP64 run with
                  clean FP64 temporaries and against the STEP_7-rounding FP64
                  run, and of the STEP_7 run against the clean one
+  adjoint        gradient of a probe-temperature loss with respect to
+                 (stefan_number, k_ratio, t_hot) by the GPU adjoint on the
+                 adjoint workload (512^2, FP64 moments, 2000 steps,
+                 checkpoint every 100, 16 seeded probes, target trajectory
+                 at St 0.12, T_h 1.05, k_ratio 1.5), central differences
+                 beside it, and the cost of the backward pass (recompute +
+                 adjoint) per step over the forward step
   cpu_baseline   the reference's own STEP_OPENMP time step (compiled from its
                  sources into oracle/_ref) timed on this host's cores on a
                  bounded sample of the same workload; the CPU oracle (a port)
@@ -30,7 +37,7 @@ Printed JSON line (one):
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
                        [--error-steps E] [--cpu-steps S] [--no-cpu-baseline]
-                       [--no-fp64] [--calibrate-copy MiB]
+                       [--no-fp64] [--no-adjoint] [--calibrate-copy MiB]
 """
 from __future__ import annotations
 
@@ -66,6 +73,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="after the timed region, run a device-to-device copy of this size "
                          "(reference kernel for HBM counter scaling in profiles)")
@@ -191,6 +199,70 @@ def traffic_from_profile(size: int, kind: str):
         return None
 
 
+def adjoint_run(n: int = 512, nsteps: int = 2000, seg: int = 100, n_probes: int = 16):
+    """The adjoint workload: gradient at the reference's parameters (St 0.1,
+    T_h = T0, k_ratio 1) of the mean squared error of T at 16 seeded probes
+    against the trajectory at St 0.12, T_h 1.05 (theta units, T0 - T_in_sol
+    = 10 K per unit above T_crit = 273), k_ratio 1.5; grav for Ra = 1e6 (the
+    2048^2 config's Rayleigh number)."""
+    import numpy as np
+    from bgk_pcm_amd.adjoint import seeded_probes
+    from bgk_pcm_amd.lattice import Lattice
+    from bgk_pcm_amd.params import FP64_MOM, ROUND_CLEAN, Params, grav_for_rayleigh
+    base = Params(nx=n, ny=n, length=float(n), rounding=ROUND_CLEAN, precision=FP64_MOM,
+                  stefan_number=0.1)
+    base = base.replace(grav=grav_for_rayleigh(base, 1e6))
+    dT = base.t0 - base.t_in_sol
+    pi, pj = seeded_probes(n, n, n_probes, seed=0)
+
+    def forward(theta, keep=False):
+        st, kr, th = theta
+        lat = Lattice(base.replace(stefan_number=st), initialize=False, t_hot=th, k_ratio=kr)
+        lat.initialization()
+        lat.adj_setup(nsteps, seg, pi, pj)
+        t0 = time.perf_counter()
+        rec = lat.adj_forward()
+        tf = time.perf_counter() - t0
+        if keep:
+            return lat, rec, tf
+        lat.close()
+        return rec
+
+    target = forward((0.12, 1.5, 273.0 + 1.05 * dT))
+    theta0 = np.array([0.1, 1.0, base.t0])
+    lat, rec, t_fwd = forward(theta0, keep=True)
+    seeds = 2.0 * (rec - target) / rec.size
+    t0 = time.perf_counter()
+    grad = lat.adj_backward(seeds)
+    t_bwd = time.perf_counter() - t0
+    lat.close()
+    # plain forward steps of the same handle type, for the cost ratio
+    with Lattice(base) as lat:
+        lat.step(50)
+        lat.synchronize()
+        t0 = time.perf_counter()
+        lat.step(nsteps)
+        lat.synchronize()
+        t_plain = time.perf_counter() - t0
+    loss = lambda r: float(((r - target) ** 2).mean())
+    fd = []
+    for c, h in enumerate((1e-6, 1e-6, 1e-6 * dT)):
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[c] += h
+        tm[c] -= h
+        fd.append((loss(forward(tp)) - loss(forward(tm))) / (2 * h))
+    rel = [abs(g - f) / abs(f) for g, f in zip(grad, fd)]
+    return {"workload": f"{n}x{n} cavity, FP64 moments, {nsteps} steps, checkpoint every {seg}, "
+                        f"{n_probes} probes; gradient at (St, k_ratio, t_hot) = "
+                        f"({theta0[0]}, {theta0[1]}, {theta0[2]})",
+            "loss": loss(rec), "grad": [float(g) for g in grad], "central_difference": fd,
+            "fd_steps": [1e-6, 1e-6, 1e-6 * dT], "rel_error": rel,
+            "forward_ms_per_step": round(t_plain / nsteps * 1e3, 5),
+            "recorded_forward_ms_per_step": round(t_fwd / nsteps * 1e3, 5),
+            "backward_ms_per_step": round(t_bwd / nsteps * 1e3, 5),
+            "backward_over_forward": round(t_bwd / t_plain, 3)}
+
+
 def measure(p, stream, steps, warmup, kind, kernel):
     """MLUPS from the wall clock and roofline figures of the fused kernel
     from HIP events on its stream."""
@@ -272,6 +344,8 @@ def main():
             res["fp32_population"]["mlups"] / res["fp64"]["mlups"], 3)
     if args.error_steps > 0:
         res["fp32_error"] = fp32_error(p, {"mom": pm, "fp32_population": pp}, args.error_steps)
+    if not args.no_adjoint:
+        res["adjoint"] = adjoint_run()
     if args.calibrate_copy > 0:
         # copies of known size for counter scaling: the runtime's device copy,
         # then 8-byte plain and non-temporal copies (libbgk_calib.so), the

@@ -227,8 +227,10 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S
   lrho += -(kCs2 + o.u * o.u) * lqfxx - (kCs2 + o.v * o.v) * lqfyy - o.u * o.v * lqfxy;
   lu += -2.0 * o.rho * o.u * lqfxx - o.rho * o.v * lqfxy;
   lv += -2.0 * o.rho * o.v * lqfyy - o.rho * o.u * lqfxy;
-  // omega_g(phi) (conductivity-ratio extension)
-  if (c.k_ratio != 1.0) {
+  // omega_g(phi, k_ratio) = 1 / (1/2 + alpha/cs2), alpha = alpha_l (fl +
+  // k_ratio (1 - fl)): the k_ratio derivative is taken at k_ratio = 1 too,
+  // where the forward uses the constant omega_g (the same value)
+  {
     const double fl = 0.5 * (1.0 + o.phi);
     const double domg_dalpha = -o.omega_g * o.omega_g / kCs2;
     lphi += lomg * domg_dalpha * c.alpha_l * (1.0 - c.k_ratio) * 0.5;

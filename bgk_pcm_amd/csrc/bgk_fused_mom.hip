@@ -360,11 +360,12 @@ __global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* ph
   phip[q] = s0.phi;
 }
 
-// Row strips of about `target` rows.  The strip count is chosen, where one
+// Row strips of about `target` rows, and at least one workgroup per slot
+// of the device (slots = CUs x resident workgroups per CU) on small
+// lattices, down to 2 rows per strip.  The strip count is chosen, where one
 // exists within a factor two, as a multiple of 8 (the XCDs) whose workgroups
-// fill whole rounds of the device (slots = CUs x resident workgroups per
-// CU): measured at 4096^2, 64-row strips (512 workgroups) run at 57 GLUPS
-// and 96-row strips (43 strips, 1.5 rounds) at 43.
+// fill whole rounds of the device: measured at 4096^2, 64-row strips (512
+// workgroups) run at 57 GLUPS and 96-row strips (43 strips, 1.5 rounds) at 43.
 StripGrid make_strip_grid(const bgk_handle* h, int w, int target, int slots) {
   StripGrid s;
   const int ny = h->ny;
@@ -372,14 +373,16 @@ StripGrid make_strip_grid(const bgk_handle* h, int w, int target, int slots) {
   if (h->strip_rows > 0) {
     s.rows = h->strip_rows;
   } else {
-    const int base = std::max(1, (ny + target - 1) / target);
-    int pick = base;
-    for (int n = base; n <= 2 * base && n <= ny; ++n)
-      if (n % NXCD == 0 && slots > 0 && ((int64_t)n * s.nseg) % slots == 0) {
+    int want = std::max(1, (ny + target - 1) / target);
+    if (slots > 0) want = std::max(want, (slots + s.nseg - 1) / s.nseg);
+    want = std::min(want, std::max(1, (ny + 1) / 2));
+    int pick = (want + NXCD - 1) / NXCD * NXCD;
+    for (int n = pick; n <= 2 * want && n <= ny; n += NXCD)
+      if (slots > 0 && ((int64_t)n * s.nseg) % slots == 0) {
         pick = n;
         break;
       }
-    s.rows = (ny + pick - 1) / pick;
+    s.rows = std::max(1, (ny + pick - 1) / pick);
   }
   s.nstrip = (ny + s.rows - 1) / s.rows;
   s.nmain = ((s.nstrip + NXCD - 1) / NXCD) * NXCD * s.nseg;

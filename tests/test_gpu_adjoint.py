@@ -32,8 +32,11 @@ def gpu_record(p, theta, pi, pj, nsteps, seg):
         return lat.adj_forward()
 
 
-@pytest.mark.parametrize("tau_f,seg", [(1.0, 7), (0.8, 50), (1.0, 1)])
-def test_adjoint_equals_autograd_of_the_cpu_reference(tau_f, seg):
+@pytest.mark.parametrize("tau_f,seg,k_ratio", [(1.0, 7, 1.5), (0.8, 50, 1.5), (1.0, 1, 1.5),
+                                               (1.0, 16, 1.0)])
+def test_adjoint_equals_autograd_of_the_cpu_reference(tau_f, seg, k_ratio):
+    """k_ratio = 1 is the reference's physics: the forward takes the constant
+    omega_g there, the gradient in k_ratio must not vanish."""
     p = Params(nx=30, ny=24, length=30.0, grav=2e-3, rounding=ROUND_CLEAN, tau_f=tau_f,
                stefan_number=0.1)
     nsteps = 120
@@ -41,11 +44,11 @@ def test_adjoint_equals_autograd_of_the_cpu_reference(tau_f, seg):
     # probes on the boundary-adjacent rows and columns too
     pi += [1, p.nx, 5]
     pj += [3, p.ny, 1]
-    theta0 = (0.1, 1.5, 283.5)
+    theta0 = (0.1, k_ratio, 283.5)
     target = probe_record(p, torch.tensor([0.12, 1.2, 283.2], dtype=torch.float64), pi, pj,
                           nsteps).numpy()
-    print(f"adjoint vs autograd: {p.nx}x{p.ny}, {nsteps} steps, seg_len {seg}, tau_f {tau_f}",
-          flush=True)
+    print(f"adjoint vs autograd: {p.nx}x{p.ny}, {nsteps} steps, seg_len {seg}, tau_f {tau_f}, "
+          f"k_ratio {k_ratio}", flush=True)
     # CPU reference gradient
     th = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
     rec_ref = probe_record(p, th, pi, pj, nsteps)
