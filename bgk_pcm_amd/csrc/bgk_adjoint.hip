@@ -172,7 +172,7 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, int i, int j, PullSu
       rebuild<double, EQF>(k, s.a[h], static_cast<const V4*>(s.b)[h],
                            c_at<double, EQF>(s.c, h), &fk, &gk);
     } else {
-      gk = ghost_gdev(k, si == 0 ? a.c.tin : s.te[sjc], a.uh[h], a.vh[h]);
+      gk = ghost_gdev(k, si == 0 ? a.c.tin : east_T(a.c, s.te, sjc), a.uh[h], a.vh[h]);
     }
     if (si < 1 || si > nx || sj < 1 || sj > ny) {
       if (k == 6 && si == nx + 1 && sj == 0)
@@ -347,8 +347,9 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
             }
           }
       }
-      // east ghost: equilibrium at this node's averaged T of step n
-      if (i == nx) {
+      // east ghost: equilibrium at this node's averaged T of step n (not
+      // with the cold-wall extension: then a constant)
+      if (i == nx && !a.c.east_cold) {
         for (int xj = j - 1; xj <= j + 1; ++xj) {
           if (xj < 1 || xj > ny) continue;
           const int64_t x = (int64_t)xj * P + nx;

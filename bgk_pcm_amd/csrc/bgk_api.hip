@@ -168,6 +168,10 @@ void set_constants(bgk_handle* h) {
   c.t_liquid0 = c.tin;
   c.k_ratio = h->ext.k_ratio;
   c.alpha_l = c.cs2 * (p.tau_g - 0.5);
+  c.east_cold = h->ext.east_cold ? 1 : 0;
+  c.t_cold = h->ext.t_cold;
+  c.noise_amp = h->ext.noise_amp * (p.t0 - p.t_in_sol);
+  c.noise_seed = h->ext.noise_seed;
   const double kvisc = c.cs2 * (p.tau_f - 0.5);
   c.u0 = p.re * kvisc / p.length;
   c.v0 = 0.0;
@@ -328,7 +332,7 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
   h->adj = nullptr;
   h->cur = 0;
   h->pc = 0;
-  h->ext.t_hot = 0.0;
+  std::memset(&h->ext, 0, sizeof h->ext);
   h->ext.k_ratio = 1.0;
   // tiling of the moment kernel: BGK_MOM_ROWS (rows per strip; default
   // about 64, or 32 for tau_f != 1, fitted to the device) and BGK_MOM_VARIANT (1..8: workgroup width 64/128/256/512 with a 4- or
@@ -405,11 +409,15 @@ int bgk_set_extensions(bgk_handle* h, const bgk_ext* e) {
   int rc = check_handle(h);
   if (rc) return rc;
   if (!e) return fail(BGK_E_INVALID, "null extensions");
-  if (!(e->t_hot >= 0.0) || !(e->k_ratio > 0.0))
-    return fail(BGK_E_INVALID, "t_hot=%g k_ratio=%g: need t_hot >= 0 and k_ratio > 0", e->t_hot,
-                e->k_ratio);
+  if (!(e->t_hot >= 0.0) || !(e->k_ratio > 0.0) || !(e->noise_amp >= 0.0) ||
+      (e->east_cold != 0 && e->east_cold != 1) || (e->east_cold && !(e->t_cold > 0.0)))
+    return fail(BGK_E_INVALID,
+                "t_hot=%g k_ratio=%g east_cold=%d t_cold=%g noise_amp=%g: need t_hot >= 0, "
+                "k_ratio > 0, east_cold 0/1 with t_cold > 0, noise_amp >= 0",
+                e->t_hot, e->k_ratio, e->east_cold, e->t_cold, e->noise_amp);
   const bool mom = h->prm.precision == BGK_FP32_MOM || h->prm.precision == BGK_FP64_MOM;
-  const bool off = (e->t_hot == 0.0 || e->t_hot == h->prm.t0) && e->k_ratio == 1.0;
+  const bool off = (e->t_hot == 0.0 || e->t_hot == h->prm.t0) && e->k_ratio == 1.0 &&
+                   !e->east_cold && e->noise_amp == 0.0;
   if (!mom && !off)
     return fail(BGK_E_UNSUPPORTED, "physics extensions need a moment handle "
                                    "(precision BGK_FP32_MOM or BGK_FP64_MOM)");

@@ -110,7 +110,29 @@ struct Consts {
   double k_ratio;      // solid/liquid conductivity ratio
   double alpha_l;      // liquid thermal diffusivity cs2 (tau_g - 1/2)
   double inv_stefan;   // 1/stefan_number (adjoint)
+  int east_cold;       // east wall at t_cold instead of the averaged T(Nx, j)
+  double t_cold;
+  double noise_amp;    // initial T noise amplitude in kelvin (0: none)
+  uint64_t noise_seed;
 };
+
+// Uniform [0,1) number of node q for the initial noise: SplitMix64 output
+// for the state seed + (q+1)*0x9E3779B97F4A7C15 (Steele, Lea and Flood's
+// generator with Vigna's constants), so that every node's value is
+// independent of the traversal order and equal on host and device.
+__host__ __device__ inline double node_uniform(uint64_t seed, int64_t q) {
+  uint64_t z = seed + (uint64_t)(q + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Temperature of the east ghost column for row j (BC_g.f90:21-31: the
+// averaged T(Nx, j) of the previous step, or the cold-wall extension).
+__device__ __forceinline__ double east_T(const Consts& c, const double* te, int j) {
+  return c.east_cold ? c.t_cold : te[j];
+}
 
 // XCD-aware tiling of the fused kernels.  The lattice interior is cut into
 // tiles of one row j and TILE_W consecutive columns.  gfx950 dispatches

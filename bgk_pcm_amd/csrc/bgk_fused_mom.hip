@@ -251,7 +251,7 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
       rebuild<R, EQF>(k, in.a[h], b_at<R, EQF>(in.b, h), c_at<R, EQF>(in.c, h), &f[k], &gd[k]);
     } else {
       const int64_t h = (int64_t)sjc * P + si;
-      gd[k] = ghost_gdev(k, si == 0 ? c.tin : te_in[sjc], m.u[h], m.v[h]);
+      gd[k] = ghost_gdev(k, si == 0 ? c.tin : east_T(c, te_in, sjc), m.u[h], m.v[h]);
     }
     if (si < 1 || si > nx || sj < 1 || sj > ny) {
       if (k == 6 && si == nx + 1 && sj == 0)
@@ -306,16 +306,16 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
   }
   if (e_edge) {
     const int64_t h = r0 + nx + 1;
-    m.T[h] = te_in[j];
+    m.T[h] = east_T(c, te_in, j);
     pb[h] = pa[h];
     te_out[j] = o.Tav;
     if (s_edge) {
-      m.T[nx + 1] = te_in[1];
+      m.T[nx + 1] = east_T(c, te_in, 1);
       pb[nx + 1] = pa[P + nx + 1];
     }
     if (n_edge) {
       const int64_t hn = (int64_t)(ny + 1) * P + nx + 1;
-      m.T[hn] = te_in[ny];
+      m.T[hn] = east_T(c, te_in, ny);
       pb[hn] = pa[(int64_t)ny * P + nx + 1];
     }
   }
@@ -348,7 +348,10 @@ __global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* ph
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > gm.nx + 1 || j > gm.ny + 1) return;
   const int64_t q = (int64_t)j * gm.pitch + i;
-  const NodeInit s0 = initial_node(c, gm, i, j, interf, west_bc);
+  NodeInit s0 = initial_node(c, gm, i, j, interf, west_bc);
+  // extension: seeded noise on the interior temperature
+  if (c.noise_amp > 0.0 && i >= 1 && i <= gm.nx && j >= 1 && j <= gm.ny)
+    s0.T += c.noise_amp * (2.0 * node_uniform(c.noise_seed, (int64_t)j * (gm.nx + 2) + i) - 1.0);
   st.a[q] = make_double2(s0.rho, s0.T - kTcrit);
   static_cast<typename Vec<R>::V4*>(st.b)[q] = make4<R>((R)s0.u, (R)s0.v, R(0), R(0));
   static_cast<typename CType<R, EQF>::T*>(st.c)[q] = typename CType<R, EQF>::T{};

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define BGK_ABI_VERSION 1
+#define BGK_ABI_VERSION 2
 
 /* return codes */
 enum {
@@ -138,12 +138,22 @@ typedef struct bgk_info {
  * T0 and k_ratio = 1 give STEP_7's step).  Honoured by moment handles
  * (BGK_FP32_MOM, BGK_FP64_MOM); other handles accept only the defaults. */
 typedef struct bgk_ext {
-  double t_hot;    /* hot (west) wall and initial liquid temperature in place
-                      of T0 (0: T0); the scale T0 - T_in_sol of
-                      STEP_7/forces.f90:12-13 is unchanged                  */
-  double k_ratio;  /* solid/liquid thermal conductivity ratio: alpha(phi) =
-                      alpha_l (fl + k_ratio (1 - fl)), fl = (1 + phi)/2,
-                      tau_g = 1/2 + alpha/cs2 (alpha_l from tau_g)          */
+  double t_hot;       /* hot (west) wall and initial liquid temperature in
+                         place of T0 (0: T0); the scale T0 - T_in_sol of
+                         STEP_7/forces.f90:12-13 is unchanged               */
+  double k_ratio;     /* solid/liquid thermal conductivity ratio: alpha(phi)
+                         = alpha_l (fl + k_ratio (1 - fl)), fl = (1+phi)/2,
+                         tau_g = 1/2 + alpha/cs2 (alpha_l from tau_g)       */
+  int32_t east_cold;  /* 1: east wall at the fixed temperature t_cold
+                         (Dirichlet) in place of STEP_7's averaged T(Nx, j)
+                         of the previous step (BC_g.f90:21-31)              */
+  int32_t reserved;   /* set to 0                                           */
+  double t_cold;      /* east wall temperature when east_cold = 1           */
+  double noise_amp;   /* initial interior T += noise_amp (T0 - T_in_sol)
+                         (2U - 1), U in [0,1) from SplitMix64 with state
+                         noise_seed + (q+1) 0x9E3779B97F4A7C15, q = i +
+                         (Nx+2) j; 0: none                                  */
+  uint64_t noise_seed;
 } bgk_ext;
 
 typedef struct bgk_handle bgk_handle;

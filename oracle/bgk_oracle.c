@@ -33,8 +33,11 @@ struct oracle {
   double cs2, omega_f, omega_g, kvisc, alpha;
   double u0, v0, tin;
   int interf;
-  /* extensions (off: t_hot = T0, k_ratio = 1) */
+  /* extensions (off: t_hot = T0, k_ratio = 1, no cold wall, no noise) */
   double t_hot, k_ratio;
+  int east_cold;
+  double t_cold, noise_amp;
+  uint64_t noise_seed;
   double* fld[BGK_NFIELDS];
 };
 
@@ -96,6 +99,28 @@ void oracle_set_ext(oracle_t* s, double t_hot, double k_ratio) {
   s->tin = t_hot;
 }
 
+/* Extensions: east wall at a fixed temperature t_cold (BC_g's east ghost
+ * equilibrium at t_cold instead of the averaged T(Nx, j)), and uniform
+ * noise on the initial interior temperature, noise_amp (T0 - T_in_sol)
+ * (2U - 1) with U the SplitMix64 output of state seed + (q+1) *
+ * 0x9E3779B97F4A7C15 for node q = i + (Nx+2) j.  Call before
+ * oracle_initialization. */
+void oracle_set_ext2(oracle_t* s, int east_cold, double t_cold, double noise_amp,
+                     uint64_t noise_seed) {
+  s->east_cold = east_cold;
+  s->t_cold = t_cold;
+  s->noise_amp = noise_amp;
+  s->noise_seed = noise_seed;
+}
+
+static double splitmix_uniform(uint64_t seed, uint64_t q) {
+  uint64_t z = seed + (q + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
 void oracle_destroy(oracle_t* s) {
   if (!s) return;
   for (int f = 0; f < BGK_NFIELDS; ++f) free(s->fld[f]);
@@ -154,6 +179,12 @@ void oracle_initialization(oracle_t* s) {
     v[q] = s->v0;
   }
   for (int j = 0; j <= ny + 1; ++j) T[IX(0, j)] = s->tin;   /* :72-74 */
+  if (s->noise_amp > 0.0) { /* extension: seeded initial noise */
+    const double a = s->noise_amp * (p->t0 - p->t_in_sol);
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i)
+        T[IX(i, j)] += a * (2.0 * splitmix_uniform(s->noise_seed, IX(i, j)) - 1.0);
+  }
   for (int i = 0; i <= nx + 1; ++i) {                        /* :76-81 */
     u[IX(i, ny + 1)] = 0.0;
     v[IX(i, ny + 1)] = 0.0;
@@ -356,7 +387,8 @@ void oracle_bc_g(oracle_t* s) {
     T[IX(0, j)] = tin;
   }
   for (int j = 1; j <= ny; ++j) { /* EAST, BC_g.f90:21-31 */
-    double ue = u[IX(nx + 1, j)], ve = v[IX(nx + 1, j)], te = T[IX(nx, j)];
+    double ue = u[IX(nx + 1, j)], ve = v[IX(nx + 1, j)];
+    double te = s->east_cold ? s->t_cold : T[IX(nx, j)]; /* cold wall: extension */
     uu = ue * ue; vv = ve * ve; uv = 1.5 * (uu + vv); upv = ue + ve; umv = ue - ve;
     g3[IX(nx + 1, j)] = w[3] * te * (1.0 - 3.0 * ue + 4.5 * uu - uv);
     g6[IX(nx + 1, j)] = w[6] * te * (1.0 - 3.0 * umv + 4.5 * umv * umv - uv);

@@ -25,6 +25,8 @@
 #ifndef BGK_ORACLE_H
 #define BGK_ORACLE_H
 
+#include <stdint.h>
+
 #include "../include/bgk_pcm.h"
 
 #ifdef __cplusplus
@@ -40,6 +42,10 @@ void oracle_destroy(oracle_t* s);
  * k_ratio = 1 give STEP_7): hot-wall / initial liquid temperature and the
  * solid/liquid conductivity ratio.  Call before oracle_initialization. */
 void oracle_set_ext(oracle_t* s, double t_hot, double k_ratio);
+/* Cold east wall at t_cold and seeded initial temperature noise (see
+ * bgk_ext in include/bgk_pcm.h). */
+void oracle_set_ext2(oracle_t* s, int east_cold, double t_cold, double noise_amp,
+                     uint64_t noise_seed);
 
 /* STEP_7/initialization.f90:5-152 */
 void oracle_initialization(oracle_t* s);

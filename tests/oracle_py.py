@@ -59,6 +59,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_diag.argtypes = [ctypes.c_void_p, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
         L.oracle_set_ext.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.oracle_set_ext2.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_uint64]
         L.oracle_tanh.restype = ctypes.c_double
         L.oracle_tanh.argtypes = [ctypes.c_double]
         _lib = L
@@ -69,7 +71,8 @@ class Oracle:
     """One CPU lattice of the reference STEP_7 (see oracle/bgk_oracle.h)."""
 
     def __init__(self, p: Params, initialize: bool = True, t_hot: float | None = None,
-                 k_ratio: float = 1.0):
+                 k_ratio: float = 1.0, t_cold: float | None = None, noise_amp: float = 0.0,
+                 noise_seed: int = 0):
         self.p = p
         self.nx, self.ny = p.nx, p.ny
         self._c = p.to_c()
@@ -78,6 +81,9 @@ class Oracle:
             raise RuntimeError("oracle_create failed")
         if t_hot is not None or k_ratio != 1.0:
             lib().oracle_set_ext(self.h, p.t0 if t_hot is None else t_hot, k_ratio)
+        if t_cold is not None or noise_amp != 0.0:
+            lib().oracle_set_ext2(self.h, 0 if t_cold is None else 1,
+                                  0.0 if t_cold is None else t_cold, noise_amp, noise_seed)
         if initialize:
             lib().oracle_initialization(self.h)
 

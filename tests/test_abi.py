@@ -36,14 +36,16 @@ def test_library_exports_every_declared_entry_point():
 
 
 def test_abi_version():
-    assert lattice.load_library().bgk_abi_version() == 1
+    assert lattice.load_library().bgk_abi_version() == 2
 
 
-def test_params_struct_layout_matches_header(tmp_path):
-    fields = [f for f, _ in BgkParamsC._fields_]
+@pytest.mark.parametrize("cname,ctype", [("bgk_params", BgkParamsC),
+                                         ("bgk_ext", lattice.BgkExtC)])
+def test_struct_layout_matches_header(tmp_path, cname, ctype):
+    fields = [f for f, _ in ctype._fields_]
     src = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"',
-           "int main(void){", 'printf("%zu\\n", sizeof(bgk_params));']
-    src += [f'printf("%zu\\n", offsetof(bgk_params, {f}));' for f in fields]
+           "int main(void){", f'printf("%zu\\n", sizeof({cname}));']
+    src += [f'printf("%zu\\n", offsetof({cname}, {f}));' for f in fields]
     src += ["return 0;}"]
     c = tmp_path / "layout.c"
     c.write_text("\n".join(src))
@@ -51,9 +53,9 @@ def test_params_struct_layout_matches_header(tmp_path):
     subprocess.run(["gcc", "-o", str(exe), str(c)], check=True)
     vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
                                            check=True).stdout.split()]
-    assert vals[0] == ctypes.sizeof(BgkParamsC)
+    assert vals[0] == ctypes.sizeof(ctype)
     for f, off in zip(fields, vals[1:]):
-        assert getattr(BgkParamsC, f).offset == off, f
+        assert getattr(ctype, f).offset == off, f
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")

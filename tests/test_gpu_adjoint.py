@@ -117,3 +117,39 @@ def test_adjoint_needs_an_fp64_moment_handle():
         with pytest.raises(BgkError) as e:
             lat.adj_setup(10, 5, [0], [3])
         assert e.value.code == -1
+
+
+def test_adjoint_with_cold_wall_and_noise_matches_central_differences():
+    """With the cold east wall the east ghosts no longer depend on the state;
+    the adjoint drops that path (checked against central differences)."""
+    p = Params(nx=40, ny=32, length=40.0, grav=2e-3, rounding=ROUND_CLEAN, stefan_number=0.1)
+    nsteps, seg = 200, 30
+    pi, pj = seeded_probes(p.nx, p.ny, 8, seed=2)
+    pi += [p.nx, p.nx]
+    pj += [3, 17]
+    kw = dict(t_cold=273.0, noise_amp=1e-3, noise_seed=7)
+
+    def rec_at(theta):
+        st, kr, th = theta
+        with Lattice(p.replace(precision=FP64_MOM, stefan_number=st), initialize=False,
+                     t_hot=th, k_ratio=kr, **kw) as lat:
+            lat.initialization()
+            lat.adj_setup(nsteps, seg, pi, pj)
+            return lat.adj_forward()
+
+    target = rec_at((0.12, 1.2, 283.2))
+    theta0 = np.array([0.1, 1.5, 283.5])
+    with Lattice(p.replace(precision=FP64_MOM, stefan_number=theta0[0]), initialize=False,
+                 t_hot=theta0[2], k_ratio=theta0[1], **kw) as lat:
+        lat.initialization()
+        lat.adj_setup(nsteps, seg, pi, pj)
+        rec = lat.adj_forward()
+        g = lat.adj_backward(2.0 * (rec - target) / rec.size)
+    loss = lambda r: float(((r - target) ** 2).mean())
+    for c, h in enumerate((1e-6, 1e-6, 1e-4)):
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[c] += h
+        tm[c] -= h
+        fd = (loss(rec_at(tp)) - loss(rec_at(tm))) / (2 * h)
+        print(f"component {c}: adjoint {g[c]:.10e} central difference {fd:.10e}")
+        assert abs(fd - g[c]) <= 1e-4 * abs(g[c]), (c, fd, g[c])

@@ -176,3 +176,20 @@ def test_extensions_need_a_moment_handle():
         with pytest.raises(BgkError) as e:
             lat.set_extensions(284.0, 0.0)
         assert e.value.code == -1
+
+
+@pytest.mark.parametrize("precision,tol", [(FP64_MOM, 1e-10), (FP32_MOM, 1e-6)])
+def test_cold_wall_and_noise_match_oracle(precision, tol):
+    """Config 1's literal physics: cold east wall, seeded initial noise
+    (amplitude 1e-3 in theta units), conductivity ratio 1."""
+    p = Params(nx=48, ny=40, length=48.0, grav=1e-3, rounding=ROUND_CLEAN, tau_f=0.8)
+    kw = dict(t_cold=273.0, noise_amp=1e-3, noise_seed=12345)
+    o = Oracle(p, **kw)
+    with Lattice(p.replace(precision=precision), **kw) as lat:
+        for f in ("T", "rho", "phi"):
+            assert np.array_equal(lat.download(f), o.field(f)), f
+        o.step(300)
+        lat.step(300)
+        err = rel_errors(lat, o, interior=False, speed_scale=True)
+    print(f"cold wall + noise: {err}")
+    assert max(err.values()) <= tol, err

@@ -231,3 +231,45 @@ def test_conductivity_ratio_scales_the_thermal_relaxation(k_ratio):
     expect = 1 - 1 / (0.5 + 3 * alpha)
     inner = (slice(1, -1), slice(1, -1))
     np.testing.assert_allclose((after / before)[inner], expect[inner], rtol=1e-9)
+
+
+def test_initial_noise_is_splitmix64():
+    """Extension: the noise generator is SplitMix64 (known answer: from state
+    0 its first output is 0xE220A8397B1DCDAF); node q = i + (Nx+2) j draws
+    the output of state seed + (q+1)*0x9E3779B97F4A7C15."""
+    first = 0xE220A8397B1DCDAF
+    p = Params(nx=6, ny=5, t0=283.0, t_in_sol=273.0)
+    base = Oracle(p)
+    amp = 1e-3
+    o = Oracle(p, noise_amp=amp, noise_seed=0)
+    dT = o.field("T") - base.field("T")
+    # node q = 0 is the corner ghost (no noise); with seed = -0x9E37..15 (mod
+    # 2^64) node q = 1 draws the first output of state 0
+    o2 = Oracle(p, noise_amp=amp, noise_seed=(-0x9E3779B97F4A7C15) % 2 ** 64)
+    u1 = (first >> 11) * 2.0 ** -53
+    d2 = o2.field("T") - base.field("T")
+    # q = 1 is (i=1, j=0): a ghost row, no noise; use q = (Nx+2)*1 + 1 instead
+    q = (p.nx + 2) * 1 + 1
+    o3 = Oracle(p, noise_amp=amp, noise_seed=(-(q) * 0x9E3779B97F4A7C15) % 2 ** 64)
+    d3 = o3.field("T") - base.field("T")
+    # T = 273 + noise, read back as a difference: exact to the rounding of 273 + x
+    assert abs(d3[1, 1] - amp * (p.t0 - p.t_in_sol) * (2 * u1 - 1)) <= 1e-13
+    inner = dT[1:-1, 1:-1]
+    assert np.all(np.abs(inner) <= amp * 10.0) and np.abs(inner).max() > 0
+    assert np.all(dT[0, :] == 0) and np.all(dT[:, 0] == 0) and np.all(d2[0, :] == 0)
+
+
+def test_cold_east_wall_sets_the_east_ghost_equilibrium():
+    """Extension: with a cold east wall, BC_g's east ghosts are equilibria
+    at t_cold with the halo velocities (not at the averaged T(Nx, j))."""
+    p = Params(nx=10, ny=8, re=2.0, length=10.0)
+    o = Oracle(p, t_cold=270.0)
+    o.step(5)
+    o.stage("bc_f")
+    o.stage("bc_g")
+    u, v = o.field("u")[1:-1, -1], o.field("v")[1:-1, -1]
+    uu = 1.5 * (u * u + v * v)
+    for k in (3, 6, 7):
+        cu = CX[k] * u + CY[k] * v
+        eq = W[k] * 270.0 * (1 + 3 * cu + 4.5 * cu * cu - uu)
+        np.testing.assert_allclose(o.field(f"g{k}")[1:-1, -1], eq, rtol=1e-15)
