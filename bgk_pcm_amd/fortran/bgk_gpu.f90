@@ -475,4 +475,158 @@ contains
     close(un)
   end subroutine out2d
 
+  ! Split binary VTK files of STEP_CUF (out2d.phi.f90, out2d.rho.f90,
+  ! out2d.temp.f90, out2d.vel.f90): ASCII header, then the interior nodes
+  ! (j outer, i inner) as big-endian real(8) stream records.
+  subroutine out2d_bin(it)
+    integer, intent(in) :: it
+    real(c_double), allocatable :: a(:, :), b(:, :)
+    allocate(a(0:prm%nx+1, 0:prm%ny+1), b(0:prm%nx+1, 0:prm%ny+1))
+    call download(F_PHI, a)
+    call bin_scalar('Outphi_', it, 'SCALARS phi double', a)
+    call download(F_RHO, a)
+    call bin_scalar('Outrho_', it, 'SCALARS rho double', a)
+    call download(F_T, a)
+    call bin_scalar('OutTem_', it, 'SCALARS Temperature double', a)
+    call download(F_U, a)
+    call download(F_V, b)
+    call bin_vel(it, a, b)
+    deallocate(a, b)
+  end subroutine out2d_bin
+
+  subroutine bin_header(un, fname, it, title)
+    integer, intent(out) :: un
+    character(len=25), intent(inout) :: fname
+    integer, intent(in) :: it
+    character(len=*), intent(in) :: title
+    write(fname(8:15), '(i8.8)') it
+    open(newunit=un, file=fname, status='unknown')
+    write(un, '(A)') '# vtk DataFile Version 2.0'
+    write(un, '(A)') 'Campo'
+    write(un, '(A)') 'BINARY'
+    write(un, '(A)') 'DATASET STRUCTURED_POINTS'
+    write(un, '(A,3I8)') 'DIMENSIONS', prm%nx, prm%ny, 1
+    write(un, '(A7,I10,A1,I10,A1,I10)') 'ORIGIN ', 1, ' ', 1, ' ', 1
+    write(un, '(A8,I10,A1,I10,A1,I10)') 'SPACING ', 1, ' ', 1, ' ', 1
+    write(un, '(A10,I10)') 'POINT_DATA ', prm%nx * prm%ny
+    write(un, '(A)') title
+  end subroutine bin_header
+
+  subroutine bin_scalar(prefix, it, title, a)
+    character(len=*), intent(in) :: prefix, title
+    integer, intent(in) :: it
+    real(c_double), intent(in) :: a(0:prm%nx+1, 0:prm%ny+1)
+    character(len=25) :: fname
+    integer :: un, i, l
+    fname = prefix // 'xxxxxxxx.vtk'
+    call bin_header(un, fname, it, title)
+    write(un, '(A)') 'LOOKUP_TABLE default'
+    close(un)
+    open(newunit=un, file=fname, status='old', position='append', form='unformatted', &
+         access='stream', convert='BIG_ENDIAN')
+    do l = 1, prm%ny
+       do i = 1, prm%nx
+          write(un) a(i, l)
+       end do
+    end do
+    close(un)
+  end subroutine bin_scalar
+
+  subroutine bin_vel(it, a, b)
+    integer, intent(in) :: it
+    real(c_double), intent(in) :: a(0:prm%nx+1, 0:prm%ny+1), b(0:prm%nx+1, 0:prm%ny+1)
+    character(len=25) :: fname
+    integer :: un, i, l
+    fname = 'Outvel_xxxxxxxx.vtk'
+    call bin_header(un, fname, it, 'VECTORS velocity double')
+    close(un)
+    open(newunit=un, file=fname, status='old', position='append', form='unformatted', &
+         access='stream', convert='BIG_ENDIAN')
+    do l = 1, prm%ny
+       do i = 1, prm%nx
+          write(un) a(i, l), b(i, l), 0.0d0
+       end do
+    end do
+    close(un)
+  end subroutine bin_vel
+
+  ! Restart file: STEP_7/dump_out.f90's records (it; per node u, v, rho,
+  ! f0..f8 and T, g0..g8, list-directed), followed by phi of every node,
+  ! which the reference's restart does not keep (its dump_in restores no
+  ! phi, so a run restarted from it melts from phi = 0).  A reader of the
+  ! reference format reads the first part unchanged.
+  subroutine dump_out(it)
+    integer, intent(in) :: it
+    real(c_double), allocatable :: a(:, :, :)
+    integer :: un, i, l, k
+    allocate(a(0:prm%nx+1, 0:prm%ny+1, 23))
+    call download(F_U, a(:, :, 1))
+    call download(F_V, a(:, :, 2))
+    call download(F_RHO, a(:, :, 3))
+    do k = 0, 8
+       call download(F_F0 + k, a(:, :, 4 + k))
+       call download(F_G0 + k, a(:, :, 14 + k))
+    end do
+    call download(F_T, a(:, :, 13))
+    call download(F_PHI, a(:, :, 23))
+    open(newunit=un, file='dumpFile.dat')
+    write(un, *) it
+    do l = 0, prm%ny + 1
+       do i = 0, prm%nx + 1
+          write(un, *) a(i, l, 1), a(i, l, 2), a(i, l, 3), a(i, l, 4), a(i, l, 5), a(i, l, 6), &
+                       a(i, l, 7), a(i, l, 8), a(i, l, 9), a(i, l, 10), a(i, l, 11), a(i, l, 12)
+          write(un, *) a(i, l, 13), a(i, l, 14), a(i, l, 15), a(i, l, 16), a(i, l, 17), &
+                       a(i, l, 18), a(i, l, 19), a(i, l, 20), a(i, l, 21), a(i, l, 22)
+       end do
+    end do
+    do l = 0, prm%ny + 1
+       do i = 0, prm%nx + 1
+          write(un, *) a(i, l, 23)
+       end do
+    end do
+    close(un)
+    deallocate(a)
+  end subroutine dump_out
+
+  ! Restart from dumpFile.dat (dump_out above; STEP_7/dump_in.f90 reads the
+  ! same first part).  Stops when the phi block is missing.
+  subroutine dump_in(it)
+    integer, intent(out) :: it
+    real(c_double), allocatable :: a(:, :, :)
+    integer :: un, i, l, k, ios
+    allocate(a(0:prm%nx+1, 0:prm%ny+1, 23))
+    open(newunit=un, file='dumpFile.dat', status='old')
+    read(un, *) it
+    do l = 0, prm%ny + 1
+       do i = 0, prm%nx + 1
+          read(un, *) a(i, l, 1), a(i, l, 2), a(i, l, 3), a(i, l, 4), a(i, l, 5), a(i, l, 6), &
+                      a(i, l, 7), a(i, l, 8), a(i, l, 9), a(i, l, 10), a(i, l, 11), a(i, l, 12)
+          read(un, *) a(i, l, 13), a(i, l, 14), a(i, l, 15), a(i, l, 16), a(i, l, 17), &
+                      a(i, l, 18), a(i, l, 19), a(i, l, 20), a(i, l, 21), a(i, l, 22)
+       end do
+    end do
+    do l = 0, prm%ny + 1
+       do i = 0, prm%nx + 1
+          read(un, *, iostat=ios) a(i, l, 23)
+          if (ios /= 0) then
+             write(0, *) 'bgk_thermal_gpu: dumpFile.dat has no phi block (a restart without'
+             write(0, *) 'the phase field would not continue the run)'
+             stop 1
+          end if
+       end do
+    end do
+    close(un)
+    call upload(F_U, a(:, :, 1))
+    call upload(F_V, a(:, :, 2))
+    call upload(F_RHO, a(:, :, 3))
+    do k = 0, 8
+       call upload(F_F0 + k, a(:, :, 4 + k))
+       call upload(F_G0 + k, a(:, :, 14 + k))
+    end do
+    call upload(F_T, a(:, :, 13))
+    call upload(F_PHI, a(:, :, 23))
+    call upload(F_PHI + 1, a(:, :, 23))
+    deallocate(a)
+  end subroutine dump_in
+
 end module bgk_gpu

@@ -17,6 +17,10 @@
 !   --fp32     FP32 population storage
 !   --mom      FP32 moment storage (rho and T in FP64; clean rounding)
 !   --clean    FP64 temporaries where STEP_7 has implicit REAL(4) ones
+!   --binary-vtk  write the split binary VTK files of STEP_CUF (Outphi_,
+!              Outrho_, OutTem_, Outvel_) instead of the ASCII Output_ file
+!   --dump-out write dumpFile.dat at the end of the run (restart: dump = 1
+!              in input.dat reads it instead of the initialization)
 program bgk_thermal_gpu
   use iso_c_binding
   use bgk_gpu
@@ -26,7 +30,7 @@ program bgk_thermal_gpu
   integer :: it, it0, iter, deltaOut, itOut, isignal, ev, s, k, nargs
   integer :: icountT0, icountT1, icount_rate, icount_max, icountL0, icountL1
   real(8) :: time_inn_loop, mem_mb, dev_mb
-  logical :: per_stage
+  logical :: per_stage, bin_vtk, dump_at_end
   character(len=64) :: arg
 
   write(6,*) "===================================="
@@ -35,6 +39,8 @@ program bgk_thermal_gpu
 
   call read_input('input.dat', p)
   per_stage = .false.
+  bin_vtk = .false.
+  dump_at_end = .false.
   nargs = command_argument_count()
   do k = 1, nargs
      call get_command_argument(k, arg)
@@ -48,6 +54,10 @@ program bgk_thermal_gpu
         p%rounding = 1
      case ('--clean')
         p%rounding = 1
+     case ('--binary-vtk')
+        bin_vtk = .true.
+     case ('--dump-out')
+        dump_at_end = .true.
      case default
         write(0,*) 'bgk_thermal_gpu: unknown option ', trim(arg)
         stop 2
@@ -66,14 +76,18 @@ program bgk_thermal_gpu
      write(150,*) '===================='
      close(150)
   else
-     write(0,*) 'bgk_thermal_gpu: dump=1 (restart) is not supported'
-     stop 1
+     call dump_in(it)
+     it0 = it + 1
   end if
 
   deltaOut = int(iter / p%noutput)
   if (it == 0) then
      itOut = deltaOut
-     call out2d(it)
+     if (bin_vtk) then
+        call out2d_bin(it)
+     else
+        call out2d(it)
+     end if
   else
      itOut = it + deltaOut
   end if
@@ -123,7 +137,11 @@ program bgk_thermal_gpu
      end if
 
      if (it == itOut .or. it == iter) then
-        call out2d(it)
+        if (bin_vtk) then
+           call out2d_bin(it)
+        else
+           call out2d(it)
+        end if
         open(unit=160, file='Nusselt.dat')
         do k = 1, 40
            write(160, "(I8,A,F10.4,A,F10.4,A,F10.4)") k, ' ', 0.0d0, ' ', 0.0d0, ' ', 0.0d0
@@ -145,6 +163,7 @@ program bgk_thermal_gpu
   call synchronize()
   call SYSTEM_CLOCK(icountT1, icount_rate, icount_max)
   time_inn_loop = real(icountT1 - icountT0) / (icount_rate)
+  if (dump_at_end) call dump_out(iter)
   call memory_report(mem_mb, dev_mb)
   write(6,*) "=========================================="
   write(6,*) "INFO: Run completed"

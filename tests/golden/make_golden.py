@@ -20,6 +20,12 @@ Each fixture holds:
   fields      names of the F stored fields (a subset of the 46)
   text_<fn>   text files written by the reference diagnostics
 
+formats_cav24x20.npz holds what the reference's output routines write
+after 50 steps of a 24x20 cavity (oracle/_ref/vtkbin/ref_vtkbin): the split
+binary VTK files of STEP_CUF as bytes (bin_<name>, uint8) and STEP_7's
+dumpFile.dat as text (text_dumpFile), with `params` and `nsteps`
+(`python tests/golden/make_golden.py formats` writes only this one).
+
 driver_cav24x20.npz instead holds every file the complete reference program
 (oracle/_ref/step7/bgk_thermal, its own main.f90) writes for a 24x20,
 2000-step input, as text: file_<name> per file, the input.dat records in
@@ -93,7 +99,39 @@ def driver_fixture():
     return path
 
 
+FORMATS_CASE = ("formats_cav24x20", Params(nx=24, ny=20, length=24.0, grav=1e-3), 50)
+
+
+def formats_fixture():
+    name, p, nsteps = FORMATS_CASE
+    exe = REF_DIR / "vtkbin" / "ref_vtkbin"
+    with tempfile.TemporaryDirectory(prefix="bgkfmt_") as wd:
+        write_input_dat(p, os.path.join(wd, "input.dat"))
+        subprocess.run([str(exe), str(nsteps)], cwd=wd, capture_output=True, text=True,
+                       timeout=600, check=True)
+        data = {"params": np.array(json.dumps(asdict(p))), "nsteps": np.array(nsteps)}
+        for fn in sorted(os.listdir(wd)):
+            if fn.endswith(".vtk"):
+                data["bin_" + fn] = np.frombuffer(open(os.path.join(wd, fn), "rb").read(),
+                                                  dtype=np.uint8)
+        data["text_dumpFile"] = np.array(open(os.path.join(wd, "dumpFile.dat")).read())
+    path = HERE / f"{name}.npz"
+    np.savez_compressed(path, **data)
+    print(f"{name}: {path.stat().st_size / 1024:.0f} KiB")
+    return path
+
+
+def update_manifest(path) -> None:
+    mf = HERE / "MANIFEST.sha256"
+    lines = [l for l in mf.read_text().splitlines() if not l.endswith("  " + path.name)]
+    lines.append(f"{hashlib.sha256(path.read_bytes()).hexdigest()}  {path.name}")
+    mf.write_text("\n".join(lines) + "\n")
+
+
 def main() -> None:
+    if len(sys.argv) > 1 and sys.argv[1] == "formats":
+        update_manifest(formats_fixture())
+        return
     manifest = []
     for name, (p, nsteps, window, stages, fields, diag_every) in CASES.items():
         variant = "step7_r8" if p.rounding == 1 else "step7"
@@ -122,6 +160,8 @@ def main() -> None:
         manifest.append(f"{digest}  {path.name}")
         print(f"{name}: {path.stat().st_size / 1024:.0f} KiB")
     path = driver_fixture()
+    manifest.append(f"{hashlib.sha256(path.read_bytes()).hexdigest()}  {path.name}")
+    path = formats_fixture()
     manifest.append(f"{hashlib.sha256(path.read_bytes()).hexdigest()}  {path.name}")
     (HERE / "MANIFEST.sha256").write_text("\n".join(manifest) + "\n")
 

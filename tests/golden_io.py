@@ -17,8 +17,9 @@ INTERIOR_ONLY = {f"fp{k}" for k in range(9)} | {f"gp{k}" for k in range(9)} | {
 
 
 def fixtures():
-    """State fixtures (the driver fixture, driver_*.npz, holds files only)."""
-    return sorted(p.stem for p in GOLDEN.glob("*.npz") if not p.stem.startswith("driver_"))
+    """State fixtures (driver_*.npz and formats_*.npz hold files only)."""
+    return sorted(p.stem for p in GOLDEN.glob("*.npz")
+                  if not p.stem.startswith(("driver_", "formats_")))
 
 
 class Golden:

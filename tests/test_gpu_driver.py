@@ -63,3 +63,48 @@ def test_driver_writes_the_reference_files(tmp_path, mode):
     checks = [l for l in r.stdout.splitlines() if "CHECK" in l]
     assert "\n".join(checks) == str(d["stdout_check"])
     assert "INFO: Mlups" in r.stdout
+
+
+FORMATS = np.load(ROOT / "tests" / "golden" / "formats_cav24x20.npz", allow_pickle=False)
+
+
+def test_driver_binary_vtk_is_the_reference_bytes(tmp_path):
+    """--binary-vtk writes STEP_CUF's split files; after 50 steps they equal
+    the reference writers' bytes (tests/golden/formats_cav24x20.npz)."""
+    from bgk_pcm_amd import vtk
+    p = Params(**json.loads(str(FORMATS["params"])))
+    n = int(FORMATS["nsteps"])
+    write_input_dat(p.replace(iter=n, noutput=1), tmp_path / "input.dat")
+    print(f"running {EXE.name} --binary-vtk, {n} steps", flush=True)
+    r = subprocess.run([str(EXE), "--binary-vtk"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    for kind in ("phi", "rho", "T"):
+        name = vtk.binary_vtk_name(kind, n)
+        assert (tmp_path / name).read_bytes() == FORMATS["bin_" + name].tobytes(), name
+    hdr, data = vtk.read_binary_vtk(tmp_path / vtk.binary_vtk_name("vel", n))
+    assert hdr["vector"] and data.shape == (p.ny, p.nx, 3) and np.all(data[..., 2] == 0)
+
+
+def test_driver_restart_continues_the_run(tmp_path):
+    """--dump-out then dump = 1: the restarted run writes the same VTK file
+    at its last step as the uninterrupted one (the dump keeps phi)."""
+    p = Params(nx=24, ny=20, length=24.0, grav=1e-3, iter=400, noutput=2)
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    write_input_dat(p, a / "input.dat")
+    print("uninterrupted 400 steps", flush=True)
+    r = subprocess.run([str(EXE)], cwd=a, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    write_input_dat(p.replace(iter=200, noutput=1), b / "input.dat")
+    print("200 steps + dump", flush=True)
+    r = subprocess.run([str(EXE), "--dump-out"], cwd=b, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert (b / "dumpFile.dat").exists()
+    write_input_dat(p.replace(dump=1), b / "input.dat")
+    print("restart to 400 steps", flush=True)
+    r = subprocess.run([str(EXE)], cwd=b, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert (b / "Output_00000400.vtk").read_text() == (a / "Output_00000400.vtk").read_text()
