@@ -259,21 +259,25 @@ __device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
   o.u2 = S.jx * o.irho;
   o.v2 = S.jy * o.irho;
 
-  double a = 0.5 * (1.0 + tap[0]);
-  double kp = a * a;
-  double b = 0.5 * (1.0 - tap[0]);
-  double km = b * b;
+  // sum_m ((1 +/- t_m)/2)^2 = (9 +/- 2 sum t + sum t^2) / 4
+  double st = tap[0], st2 = tap[0] * tap[0];
 #pragma unroll
   for (int m = 1; m < 9; ++m) {
-    a = 0.5 * (1.0 + tap[m]);
-    kp = kp + a * a;
-    b = 0.5 * (1.0 - tap[m]);
-    km = km + b * b;
+    st = st + tap[m];
+    st2 = fma(tap[m], tap[m], st2);
   }
+  const double kp = 0.25 * ((9.0 + st2) + 2.0 * st);
+  const double km = 0.25 * ((9.0 + st2) - 2.0 * st);
   o.kp_sum = kp;
   o.km_sum = km;
   o.pc = tap[0];
-  o.rm = 1.0 / (1.0 + exp(o.dT2 * c.two_over_tw));
+  // Both reaction terms vanish exactly when kp = 0 (all taps -1) or pc = 1,
+  // and km = 0 (all taps +1) or pc = -1: phi = pc whatever the rates, which
+  // are then not evaluated (rm = 1/2 keeps every product below finite; the
+  // adjoint multiplies its rate terms by the same exact zeros).
+  const bool react = !((kp == 0.0 || o.pc == 1.0) && (km == 0.0 || o.pc == -1.0));
+  o.rm = 0.5;
+  if (react) o.rm = 1.0 / (1.0 + exp(o.dT2 * c.two_over_tw));
   const double rp = 1.0 - o.rm;
   const double kpr = (kp * 0.125) * rp;
   const double kmr = (km * 0.125) * o.rm;
