@@ -30,6 +30,13 @@ Printed JSON line (one):
                  at St 0.12, T_h 1.05, k_ratio 1.5), central differences
                  beside it, and the cost of the backward pass (recompute +
                  adjoint) per step over the forward step
+  adjoint_large  the large adjoint workload (8192^2, FP32 moment storage,
+                 Ra 1e7, St 0.1, 5000 steps, two levels of checkpoints every
+                 289 and 17 steps, 16 seeded probes, the same kind of target
+                 trajectory): device bytes of the handle and of the
+                 adjoint's checkpoints, forward and backward time per step
+                 and their ratio; and the same gradient by the FP32 and the
+                 FP64 adjoint at 1024^2 with their relative difference
   cpu_baseline   the reference's own STEP_OPENMP time step (compiled from its
                  sources into oracle/_ref) timed on this host's cores on a
                  bounded sample of the same workload; the CPU oracle (a port)
@@ -37,7 +44,8 @@ Printed JSON line (one):
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
                        [--error-steps E] [--cpu-steps S] [--no-cpu-baseline]
-                       [--no-fp64] [--no-adjoint] [--calibrate-copy MiB]
+                       [--no-fp64] [--no-adjoint] [--no-adjoint-large]
+                       [--calibrate-copy MiB]
 """
 from __future__ import annotations
 
@@ -74,6 +82,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-adjoint-large", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="after the timed region, run a device-to-device copy of this size "
                          "(reference kernel for HBM counter scaling in profiles)")
@@ -263,6 +272,86 @@ def adjoint_run(n: int = 512, nsteps: int = 2000, seg: int = 100, n_probes: int 
             "backward_over_forward": round(t_bwd / t_plain, 3)}
 
 
+def progress(msg: str) -> None:
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
+def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: int = 17,
+                  n_probes: int = 16, check_n: int = 1024):
+    """The large adjoint workload: FP32 moment storage at n^2 (cavity of the
+    bench, Ra 1e7, St 0.1) over nsteps steps with two levels of checkpoints
+    (about nsteps^(1/3) steps apart in the inner level); the loss is the
+    mean squared error of T at seeded probes against the trajectory at St
+    0.12, T_h 1.05, k_ratio 1.5.  Then the FP32 and the FP64 adjoint of the
+    same loss at check_n^2, for the gradient check."""
+    import numpy as np
+    from bgk_pcm_amd.adjoint import seeded_probes
+    from bgk_pcm_amd.lattice import Lattice
+    from bgk_pcm_amd.params import FP32_MOM, FP64_MOM, ROUND_CLEAN
+
+    def record(base, theta, pi, pj, keep=False):
+        st, kr, th = theta
+        lat = Lattice(base.replace(stefan_number=st), initialize=False, t_hot=th, k_ratio=kr)
+        lat.initialization()
+        lat.adj_setup(nsteps, seg, pi, pj, seg_outer=seg_outer)
+        t0 = time.perf_counter()
+        rec = lat.adj_forward()
+        tf = time.perf_counter() - t0
+        if keep:
+            return lat, rec, tf
+        lat.close()
+        return rec
+
+    def gradient(base, pi, pj, target, theta0):
+        lat, rec, tf = record(base, theta0, pi, pj, keep=True)
+        t0 = time.perf_counter()
+        g = lat.adj_backward(2.0 * (rec - target) / rec.size)
+        tb = time.perf_counter() - t0
+        adj_bytes, nstates = lat.adj_info()
+        handle_bytes = lat.info()["device_bytes"]
+        lat.close()
+        return g, tf, tb, adj_bytes, nstates, handle_bytes
+
+    out = {}
+    for size, prec_list in ((n, (FP32_MOM,)), (check_n, (FP32_MOM, FP64_MOM))):
+        base = cavity(size).replace(rounding=ROUND_CLEAN, stefan_number=0.1)
+        dT = base.t0 - base.t_in_sol
+        theta0 = (0.1, 1.0, base.t0)
+        pi, pj = seeded_probes(size, size, n_probes, seed=0)
+        for prec in prec_list:
+            b = base.replace(precision=prec)
+            progress(f"adjoint {size}^2 {'FP32' if prec == FP32_MOM else 'FP64'} moments: target")
+            target = record(b, (0.12, 1.5, 273.0 + 1.05 * dT), pi, pj)
+            progress(f"adjoint {size}^2: forward and backward")
+            g, tf, tb, adj_bytes, nstates, hb = gradient(b, pi, pj, target, theta0)
+            out[(size, prec)] = (g, tf, tb, adj_bytes, nstates, hb)
+    g, tf, tb, adj_bytes, nstates, hb = out[(n, FP32_MOM)]
+    # plain forward steps of the same handle type, for the cost ratio
+    progress(f"adjoint {n}^2: plain forward steps")
+    with Lattice(cavity(n).replace(precision=FP32_MOM, rounding=ROUND_CLEAN)) as lat:
+        lat.step(20)
+        lat.synchronize()
+        t0 = time.perf_counter()
+        lat.step(500)
+        lat.synchronize()
+        t_plain = (time.perf_counter() - t0) / 500
+    g32, g64 = out[(check_n, FP32_MOM)][0], out[(check_n, FP64_MOM)][0]
+    return {"workload": f"{n}x{n} cavity, FP32 moments, Ra 1e7, St 0.1, {nsteps} steps, "
+                        f"checkpoints every {seg_outer} and {seg} steps, {n_probes} probes",
+            "grad": [float(x) for x in g],
+            "states_stored": nstates,
+            "adjoint_device_bytes": adj_bytes, "handle_device_bytes": hb,
+            "hbm_footprint_gb": round((adj_bytes + hb) / 1e9, 2),
+            "forward_ms_per_step": round(t_plain * 1e3, 4),
+            "recorded_forward_ms_per_step": round(tf / nsteps * 1e3, 4),
+            "backward_ms_per_step": round(tb / nsteps * 1e3, 4),
+            "backward_over_forward": round(tb / nsteps / t_plain, 3),
+            "check": {"size": f"{check_n}x{check_n}", "steps": nsteps,
+                      "grad_fp32_moments": [float(x) for x in g32],
+                      "grad_fp64_moments": [float(x) for x in g64],
+                      "rel_difference": [float(abs(a - b) / abs(b)) for a, b in zip(g32, g64)]}}
+
+
 def measure(p, stream, steps, warmup, kind, kernel):
     """MLUPS from the wall clock and roofline figures of the fused kernel
     from HIP events on its stream."""
@@ -343,9 +432,13 @@ def main():
         res["fp32_population_over_fp64"] = round(
             res["fp32_population"]["mlups"] / res["fp64"]["mlups"], 3)
     if args.error_steps > 0:
+        progress(f"FP32 storage errors after {args.error_steps} steps")
         res["fp32_error"] = fp32_error(p, {"mom": pm, "fp32_population": pp}, args.error_steps)
     if not args.no_adjoint:
+        progress("adjoint 512^2")
         res["adjoint"] = adjoint_run()
+    if not args.no_adjoint_large:
+        res["adjoint_large"] = adjoint_large()
     if args.calibrate_copy > 0:
         # copies of known size for counter scaling: the runtime's device copy,
         # then 8-byte plain and non-temporal copies (libbgk_calib.so), the

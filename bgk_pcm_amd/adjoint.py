@@ -9,14 +9,17 @@
     loss = ((T - T_target) ** 2).mean()
     loss.backward()          # theta.grad = dL/d(St, k_ratio, t_hot)
 
-Forward model: the FP64 moment-storage step (precision BGK_FP64_MOM, clean
-rounding of STEP_7's collision) of the no-slip cavity described by
+Forward model: the FP64 moment-storage step (precision BGK_FP64_MOM, or
+BGK_FP32_MOM through `precision`: the adjoint is then that of the step at
+the FP32-stored states; clean rounding of STEP_7's collision) of the no-slip
+cavity described by
 `base_params` (the input.dat records), with the library's extensions
 t_hot (hot-wall and initial liquid temperature in place of T0) and k_ratio
 (solid/liquid conductivity ratio).  T[n, p] is the averaged temperature of
 STEP_7/moments.f90:11-25 at interior node (probes_i[p], probes_j[p]) after
-step n+1.  The state is checkpointed every seg_len steps on the GPU and
-recomputed segment by segment in the backward pass.  The tensors live on
+step n+1.  The state is checkpointed every seg_len steps on the GPU (and
+every seg_outer steps first, with two levels) and recomputed segment by
+segment in the backward pass.  The tensors live on
 the CPU (the probe record is small); the lattice stays on the GPU.
 """
 from __future__ import annotations
@@ -33,12 +36,13 @@ from .params import FP64_MOM, ROUND_CLEAN, Params
 class ProbeTemperature(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta: torch.Tensor, base: Params, pi: Sequence[int], pj: Sequence[int],
-                nsteps: int, seg_len: int) -> torch.Tensor:
+                nsteps: int, seg_len: int, seg_outer: int = 0,
+                precision: int = FP64_MOM) -> torch.Tensor:
         st, kr, th = (float(x) for x in theta.detach().cpu().tolist())
-        p = base.replace(precision=FP64_MOM, rounding=ROUND_CLEAN, stefan_number=st)
+        p = base.replace(precision=precision, rounding=ROUND_CLEAN, stefan_number=st)
         lat = Lattice(p, initialize=False, t_hot=th, k_ratio=kr)
         lat.initialization()
-        lat.adj_setup(nsteps, seg_len, pi, pj)
+        lat.adj_setup(nsteps, seg_len, pi, pj, seg_outer=seg_outer)
         rec = lat.adj_forward()
         ctx.lat = lat
         return torch.from_numpy(rec)
@@ -49,14 +53,16 @@ class ProbeTemperature(torch.autograd.Function):
         g = lat.adj_backward(grad_out.detach().cpu().numpy().astype(np.float64))
         lat.close()
         ctx.lat = None
-        return torch.from_numpy(g), None, None, None, None, None
+        return torch.from_numpy(g), None, None, None, None, None, None, None
 
 
 def probe_temperature(theta: torch.Tensor, base: Params, pi: Sequence[int], pj: Sequence[int],
-                      nsteps: int, seg_len: int = 100) -> torch.Tensor:
+                      nsteps: int, seg_len: int = 100, seg_outer: int = 0,
+                      precision: int = FP64_MOM) -> torch.Tensor:
     """T at the probes after each of nsteps steps, differentiable in
     theta = (stefan_number, k_ratio, t_hot)."""
-    return ProbeTemperature.apply(theta, base, list(pi), list(pj), int(nsteps), int(seg_len))
+    return ProbeTemperature.apply(theta, base, list(pi), list(pj), int(nsteps), int(seg_len),
+                                  int(seg_outer), int(precision))
 
 
 def seeded_probes(nx: int, ny: int, n: int, seed: int) -> tuple[list[int], list[int]]:

@@ -1,7 +1,9 @@
 // bgk_adjoint.hip -- discrete adjoint of the moment-storage time step.
 //
-// Forward model: the FP64 moment-storage step (bgk_fused_mom.hip, clean
-// rounding, no-slip cavity, extensions t_hot and k_ratio).  Loss: any
+// Forward model: the moment-storage step (bgk_fused_mom.hip, clean rounding,
+// no-slip cavity, extensions t_hot and k_ratio) with FP64 moments, or FP32
+// moments (R = float: the adjoint of the step evaluated at the stored
+// states, the storage rounding taken as the identity).  Loss: any
 // function of the averaged temperature T (STEP_7/moments.f90:11-25) at a
 // set of probe nodes after every step; the caller supplies dL/dT(step,
 // probe) and receives dL/d(stefan_number, k_ratio, t_hot).
@@ -21,6 +23,9 @@
 // bgk_adj_backward recomputes one segment at a time from its checkpoint,
 // keeping every state of the segment, then runs the adjoint kernels over
 // the segment backwards.  Memory: (nsteps/seg_len + seg_len + 1) states.
+// With two levels the forward run keeps every seg_outer-th state, and each
+// outer segment is first recomputed to checkpoints every seg_inner steps:
+// nsteps/seg_outer + seg_outer/seg_inner + seg_inner + 1 states.
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -89,28 +94,28 @@ struct Mom {
   double rho, th, u, v, pf[3], pg[3];
 };
 
-template <bool EQF>
+template <typename R, bool EQF>
 __device__ __forceinline__ Mom load_mom(const StatePtrs& s, int64_t q) {
   Mom m;
   const double2 a = s.a[q];
-  const double4 b = static_cast<const double4*>(s.b)[q];
+  const typename Vec<R>::V4 b = b_at<R, EQF>(s.b, q);
   m.rho = a.x;
   m.th = a.y;
-  m.u = b.x;
-  m.v = b.y;
+  m.u = (double)b.x;
+  m.v = (double)b.y;
   if constexpr (EQF) {
     m.pf[0] = m.pf[1] = m.pf[2] = 0.0;
-    m.pg[0] = b.z;
-    m.pg[1] = b.w;
-    m.pg[2] = static_cast<const double*>(s.c)[q];
+    m.pg[0] = (double)b.z;
+    m.pg[1] = (double)b.w;
+    m.pg[2] = (double)c_at<R, EQF>(s.c, q);
   } else {
-    const double4 cc = static_cast<const double4*>(s.c)[q];
-    m.pf[0] = b.z;
-    m.pf[1] = b.w;
-    m.pf[2] = cc.x;
-    m.pg[0] = cc.y;
-    m.pg[1] = cc.z;
-    m.pg[2] = cc.w;
+    const typename Vec<R>::V4 cc = c_at<R, EQF>(s.c, q);
+    m.pf[0] = (double)b.z;
+    m.pf[1] = (double)b.w;
+    m.pf[2] = (double)cc.x;
+    m.pg[0] = (double)cc.y;
+    m.pg[1] = (double)cc.z;
+    m.pg[2] = (double)cc.w;
   }
   return m;
 }
@@ -153,14 +158,13 @@ __device__ __forceinline__ double ghost_dT(int k, double u, double v) {
 
 // Pulled sums and taps of step n at node (i, j) from the state before the
 // step, with the boundary folding of the forward kernels (mom_edge_nodes).
-template <bool EQF>
+template <typename R, bool EQF>
 __device__ __forceinline__ void pull_step(const AdjArgs& a, int i, int j, PullSums* S,
                                           double tap[9]) {
   const int nx = a.gm.nx, ny = a.gm.ny;
   const int64_t P = a.gm.pitch;
   const int64_t q = (int64_t)j * P + i;
   const StatePtrs& s = a.sprev;
-  using V4 = typename Vec<double>::V4;
   pull_zero(*S);
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
@@ -169,8 +173,7 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, int i, int j, PullSu
     double fk = 0.0, gk;
     const int64_t h = (int64_t)sjc * P + si;
     if (si >= 1 && si <= nx) {
-      rebuild<double, EQF>(k, s.a[h], static_cast<const V4*>(s.b)[h],
-                           c_at<double, EQF>(s.c, h), &fk, &gk);
+      rebuild<R, EQF>(k, s.a[h], b_at<R, EQF>(s.b, h), c_at<R, EQF>(s.c, h), &fk, &gk);
     } else {
       gk = ghost_gdev(k, si == 0 ? a.c.tin : east_T(a.c, s.te, sjc), a.uh[h], a.vh[h]);
     }
@@ -180,8 +183,7 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, int i, int j, PullSu
       else if (k == 7 && si == nx + 1 && sj == ny + 1)
         fk = a.c.frozen7;
       else
-        fk = rebuild_f<double, EQF>(OPP[k], s.a[q], static_cast<const V4*>(s.b)[q],
-                                    c_at<double, EQF>(s.c, q));
+        fk = rebuild_f<R, EQF>(OPP[k], s.a[q], b_at<R, EQF>(s.b, q), c_at<R, EQF>(s.c, q));
     }
     pull_add(*S, k, fk, gk);
   }
@@ -286,7 +288,7 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S
   lS->gxy = lqgxy;
 }
 
-template <bool EQF>
+template <typename R, bool EQF>
 __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
   const int nx = a.gm.nx, ny = a.gm.ny;
   const int64_t P = a.gm.pitch;
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
   double g[3] = {0.0, 0.0, 0.0};
   if (i <= nx && j <= ny) {
     const int64_t q = (int64_t)j * P + i;
-    const Mom m = load_mom<EQF>(a.sn, q);
+    const Mom m = load_mom<R, EQF>(a.sn, q);
     Mom lm = {};
     double lphi = 0.0;
     double lTrep = 0.0;
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
       // ---- reverse of step n at this node
       PullSums S;
       double tap[9];
-      pull_step<EQF>(a, i, j, &S, tap);
+      pull_step<R, EQF>(a, i, j, &S, tap);
       const Core o = core_forward(a.c, S, tap);
       PullSums lS;
       double lkp, lkm, lpc;
@@ -475,11 +477,16 @@ __global__ void k_probe_set(int* idx, const int* pij, int np, int64_t pitch) {
 // ------------------------------------------------------------------ host
 
 struct AdjCtx {
-  int nsteps = 0, seg = 0, np = 0, nckpt = 0;
+  int nsteps = 0, np = 0;
+  int seg = 0;    // inner segment: every state of one is stored
+  int seg1 = 0;   // outer segment (two levels), 0: one level
+  int nckpt = 0;  // checkpoints of the forward run (every seg1, or seg, steps)
+  int nckpt2 = 0; // checkpoints of one outer segment (two levels)
   void* pool = nullptr;
   size_t bytes = 0;
-  std::vector<StatePtrs> slot;  // seg + 1 states of a segment
-  std::vector<StatePtrs> ckpt;  // checkpoints at steps 0, seg, 2 seg, ...
+  std::vector<StatePtrs> slot;   // states 1..seg of an inner segment (slot[0] spare)
+  std::vector<StatePtrs> ckpt;   // checkpoints of the forward run
+  std::vector<StatePtrs> ckpt2;  // checkpoints of the outer segment being reversed
   AdjPlanes lam[2];
   int* probe_idx = nullptr;
   int* probe_ij = nullptr;
@@ -494,7 +501,15 @@ struct AdjCtx {
 
 namespace {
 
-size_t c_bytes(const bgk_handle* h) { return h->mom_eqf ? sizeof(double) : sizeof(double4); }
+// bytes per node of the B and C moment planes of the handle's storage
+size_t b_bytes(const bgk_handle* h) {
+  return h->prm.precision == BGK_FP64_MOM ? sizeof(double4) : sizeof(float4);
+}
+size_t c_bytes(const bgk_handle* h) {
+  const bool f64 = h->prm.precision == BGK_FP64_MOM;
+  if (h->mom_eqf) return f64 ? sizeof(double) : sizeof(float);
+  return f64 ? sizeof(double4) : sizeof(float4);
+}
 
 struct Bump {
   char* base;
@@ -511,7 +526,7 @@ struct Bump {
 void carve_state(Bump& b, const bgk_handle* h, StatePtrs* s) {
   const size_t n = (size_t)h->plane_elems;
   s->a = b.take<double2>(n);
-  s->b = b.take<double4>(n);
+  s->b = b.take<char>(n * b_bytes(h));
   s->c = b.take<char>(n * c_bytes(h));
   s->phi = b.take<double>(n);
   s->te = b.take<double>((size_t)h->ny + 2);
@@ -522,8 +537,10 @@ size_t carve_ctx(bgk_handle* h, AdjCtx* a, char* base) {
   const size_t n = (size_t)h->plane_elems;
   a->slot.assign((size_t)a->seg + 1, StatePtrs{});
   a->ckpt.assign((size_t)a->nckpt, StatePtrs{});
+  a->ckpt2.assign((size_t)a->nckpt2, StatePtrs{});
   for (auto& s : a->slot) carve_state(b, h, &s);
   for (auto& s : a->ckpt) carve_state(b, h, &s);
+  for (auto& s : a->ckpt2) carve_state(b, h, &s);
   for (int l = 0; l < 2; ++l)
     for (int v = 0; v < 13; ++v) a->lam[l].v[v] = b.take<double>(n);
   a->probe_idx = b.take<int>(n);
@@ -541,7 +558,7 @@ hipError_t copy_state(bgk_handle* h, const StatePtrs& dst, const double2* a, con
   const size_t n = (size_t)h->plane_elems;
   hipError_t e = hipMemcpyAsync(dst.a, a, n * sizeof(double2), hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(dst.b, bb, n * sizeof(double4), hipMemcpyDeviceToDevice, h->stream);
+    e = hipMemcpyAsync(dst.b, bb, n * b_bytes(h), hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(dst.c, c, n * c_bytes(h), hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess)
@@ -570,10 +587,15 @@ hipError_t launch_adjoint(bgk_handle* h, AdjCtx* x, const StatePtrs& sn, const S
   a.gm = h->geom;
   a.interf = h->nx / 40;
   dim3 grd((unsigned)((h->nx + TX - 1) / TX), (unsigned)((h->ny + TY - 1) / TY));
-  if (h->mom_eqf)
-    hipLaunchKernelGGL(k_adjoint<true>, grd, dim3(TX, TY), 0, h->stream, a);
+  const bool f64 = h->prm.precision == BGK_FP64_MOM;
+  if (h->mom_eqf && f64)
+    hipLaunchKernelGGL((k_adjoint<double, true>), grd, dim3(TX, TY), 0, h->stream, a);
+  else if (h->mom_eqf)
+    hipLaunchKernelGGL((k_adjoint<float, true>), grd, dim3(TX, TY), 0, h->stream, a);
+  else if (f64)
+    hipLaunchKernelGGL((k_adjoint<double, false>), grd, dim3(TX, TY), 0, h->stream, a);
   else
-    hipLaunchKernelGGL(k_adjoint<false>, grd, dim3(TX, TY), 0, h->stream, a);
+    hipLaunchKernelGGL((k_adjoint<float, false>), grd, dim3(TX, TY), 0, h->stream, a);
   return hipGetLastError();
 }
 
@@ -603,19 +625,24 @@ void adj_free(bgk_handle* h) {
   h->adj = nullptr;
 }
 
-hipError_t adj_setup(bgk_handle* h, int nsteps, int seg, int np, const int32_t* pi,
-                     const int32_t* pj) {
+hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg, int np,
+                     const int32_t* pi, const int32_t* pj) {
   adj_free(h);
   AdjCtx* x = new (std::nothrow) AdjCtx();
   if (!x) return hipErrorOutOfMemory;
   x->nsteps = nsteps;
   x->seg = seg;
+  x->seg1 = seg_outer;
   x->np = np;
-  x->nckpt = (nsteps + seg - 1) / seg;
+  const int lvl1 = seg_outer > 0 ? seg_outer : seg;
+  x->nckpt = (nsteps + lvl1 - 1) / lvl1;
+  // (the first checkpoint of an outer segment is the outer checkpoint)
+  x->nckpt2 = seg_outer > 0 ? (std::min(seg_outer, nsteps) + seg - 1) / seg - 1 : 0;
   x->nblk = (int)(((h->nx + TX - 1) / TX) * ((h->ny + TY - 1) / TY));
   x->bytes = carve_ctx(h, x, nullptr);
   hipError_t e = hipMalloc(&x->pool, x->bytes);
   if (e != hipSuccess) {
+    (void)hipGetLastError();  // (the failed allocation's error is reported here)
     delete x;
     return e;
   }
@@ -660,8 +687,9 @@ hipError_t adj_forward(bgk_handle* h, double* probe_T_host) {
     const int a = h->cur, b = 1 - h->cur;
     StatePtrs in{h->ma[a], h->mb[a], h->mc[a], h->phi[h->pc], h->teast[te]};
     StatePtrs out{h->ma[b], h->mb[b], h->mc[b], h->phi[1 - h->pc], h->teast[1 - te]};
-    if (n % x->seg == 0) {
-      e = copy_state(h, x->ckpt[n / x->seg], in.a, in.b, in.c, in.phi, in.te);
+    const int lvl1 = x->seg1 > 0 ? x->seg1 : x->seg;
+    if (n % lvl1 == 0) {
+      e = copy_state(h, x->ckpt[n / lvl1], in.a, in.b, in.c, in.phi, in.te);
       if (e != hipSuccess) return e;
     }
     e = fwd_step(h, in, out, x->probe_T + (size_t)n * x->np, x->probe_idx, n == x->nsteps - 1,
@@ -680,6 +708,12 @@ hipError_t adj_forward(bgk_handle* h, double* probe_T_host) {
   return e;
 }
 
+void adj_info(const bgk_handle* h, int64_t* bytes, int32_t* n_states) {
+  const AdjCtx* x = h->adj;
+  *bytes = (int64_t)x->bytes;
+  *n_states = (int32_t)(x->slot.size() + x->ckpt.size() + x->ckpt2.size());
+}
+
 hipError_t adj_backward(bgk_handle* h, const double* seeds_host, double grad[3]) {
   AdjCtx* x = h->adj;
   hipError_t e = hipSuccess;
@@ -690,23 +724,45 @@ hipError_t adj_backward(bgk_handle* h, const double* seeds_host, double grad[3])
     e = hipMemsetAsync(x->partial, 0, 3 * (size_t)x->nblk * sizeof(double), h->stream);
   int ln = 0;  // lam[ln] holds the adjoint of the step after the current one
   bool has_next = false;
-  for (int s = x->nckpt - 1; s >= 0 && e == hipSuccess; --s) {
-    const int n0 = s * x->seg;
-    const int n1 = std::min(n0 + x->seg, x->nsteps);
-    const StatePtrs& ck = x->ckpt[s];
-    e = copy_state(h, x->slot[0], ck.a, ck.b, ck.c, ck.phi, ck.te);
+  // steps n0..n1-1 from `start` (the state before step n0): recompute every
+  // state of the segment (state n0 is `start` itself), then reverse them
+  auto inner = [&](const StatePtrs& start, int n0, int n1) {
+    auto st = [&](int n) -> const StatePtrs& { return n == n0 ? start : x->slot[n - n0]; };
     for (int n = n0; n < n1 && e == hipSuccess; ++n)
-      e = fwd_step(h, x->slot[n - n0], x->slot[n - n0 + 1], nullptr, nullptr, false, x->macro);
+      e = fwd_step(h, st(n), st(n + 1), nullptr, nullptr, false, x->macro);
     for (int n = n1; n > n0 && e == hipSuccess; --n) {
       const double* sd = x->np > 0 ? x->seeds + (size_t)(n - 1) * x->np : nullptr;
-      e = launch_adjoint(h, x, x->slot[n - n0], x->slot[n - n0 - 1], ln, has_next ? 1 : 0, 1, sd);
+      e = launch_adjoint(h, x, st(n), st(n - 1), ln, has_next ? 1 : 0, 1, sd);
       ln = 1 - ln;
       has_next = true;
+    }
+  };
+  if (x->seg1 == 0) {
+    for (int s = x->nckpt - 1; s >= 0 && e == hipSuccess; --s)
+      inner(x->ckpt[s], s * x->seg, std::min((s + 1) * x->seg, x->nsteps));
+  } else {
+    for (int s1 = x->nckpt - 1; s1 >= 0 && e == hipSuccess; --s1) {
+      const int m0 = s1 * x->seg1, m1 = std::min(m0 + x->seg1, x->nsteps);
+      const int ns2 = (m1 - m0 + x->seg - 1) / x->seg;
+      // checkpoints of this outer segment: c2(k) = state m0 + k seg, c2(0)
+      // the outer checkpoint itself; the steps in between go through
+      // slot[0] and slot[1]
+      auto c2 = [&](int k) -> const StatePtrs& { return k == 0 ? x->ckpt[s1] : x->ckpt2[k - 1]; };
+      for (int k = 1; k < ns2 && e == hipSuccess; ++k) {
+        const StatePtrs* src = &c2(k - 1);
+        for (int i = 0; i < x->seg && e == hipSuccess; ++i) {
+          const StatePtrs* dst = (i == x->seg - 1) ? &x->ckpt2[k - 1] : &x->slot[i & 1];
+          e = fwd_step(h, *src, *dst, nullptr, nullptr, false, x->macro);
+          src = dst;
+        }
+      }
+      for (int s2 = ns2 - 1; s2 >= 0 && e == hipSuccess; --s2)
+        inner(c2(s2), m0 + s2 * x->seg, std::min(m0 + (s2 + 1) * x->seg, m1));
     }
   }
   // the initial state's share (hot-wall temperature of the liquid strip)
   if (e == hipSuccess)
-    e = launch_adjoint(h, x, x->slot[0], x->slot[0], ln, has_next ? 1 : 0, 0, nullptr);
+    e = launch_adjoint(h, x, x->ckpt[0], x->ckpt[0], ln, has_next ? 1 : 0, 0, nullptr);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, h->stream, x->partial, x->nblk, x->out3);
     e = hipGetLastError();

@@ -600,20 +600,39 @@ int bgk_gather(bgk_handle* h, int32_t field, int32_t n, const int32_t* i, const 
   return BGK_OK;
 }
 
-int bgk_adj_setup(bgk_handle* h, int32_t nsteps, int32_t seg_len, int32_t n_probes,
-                  const int32_t* pi, const int32_t* pj) {
+int bgk_adj_setup_levels(bgk_handle* h, int32_t nsteps, int32_t seg_outer, int32_t seg_inner,
+                         int32_t n_probes, const int32_t* pi, const int32_t* pj) {
   int rc = check_handle(h);
   if (rc) return rc;
-  if (h->prm.precision != BGK_FP64_MOM)
-    return fail(BGK_E_UNSUPPORTED, "the adjoint differentiates the FP64 moment step: create the "
-                                   "handle with precision BGK_FP64_MOM");
-  if (nsteps < 1 || seg_len < 1 || n_probes < 0 || (n_probes > 0 && (!pi || !pj)))
-    return fail(BGK_E_INVALID, "nsteps=%d seg_len=%d n_probes=%d", nsteps, seg_len, n_probes);
+  if (h->prm.precision != BGK_FP64_MOM && h->prm.precision != BGK_FP32_MOM)
+    return fail(BGK_E_UNSUPPORTED, "the adjoint differentiates the moment step: create the "
+                                   "handle with precision BGK_FP64_MOM or BGK_FP32_MOM");
+  if (nsteps < 1 || seg_inner < 1 || seg_outer < 0 || (seg_outer > 0 && seg_outer < seg_inner) ||
+      n_probes < 0 || (n_probes > 0 && (!pi || !pj)))
+    return fail(BGK_E_INVALID, "nsteps=%d seg_outer=%d seg_inner=%d n_probes=%d", nsteps,
+                seg_outer, seg_inner, n_probes);
   for (int32_t p = 0; p < n_probes; ++p)
     if (pi[p] < 1 || pi[p] > h->nx || pj[p] < 1 || pj[p] > h->ny)
       return fail(BGK_E_INVALID, "probe %d at (%d,%d) is not an interior node", p, pi[p], pj[p]);
-  hipError_t e = bgk::adj_setup(h, nsteps, seg_len, n_probes, pi, pj);
+  hipError_t e = bgk::adj_setup(h, nsteps, seg_outer, seg_inner, n_probes, pi, pj);
+  if (e == hipErrorOutOfMemory)
+    return fail(BGK_E_NOMEM, "adjoint checkpoints of %d steps (segments %d, %d) do not fit",
+                nsteps, seg_outer, seg_inner);
   if (e != hipSuccess) return hip_fail(e, "adjoint setup");
+  return BGK_OK;
+}
+
+int bgk_adj_setup(bgk_handle* h, int32_t nsteps, int32_t seg_len, int32_t n_probes,
+                  const int32_t* pi, const int32_t* pj) {
+  return bgk_adj_setup_levels(h, nsteps, 0, seg_len, n_probes, pi, pj);
+}
+
+int bgk_adj_info(bgk_handle* h, int64_t* device_bytes, int32_t* n_states) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!h->adj) return fail(BGK_E_INVALID, "bgk_adj_setup first");
+  if (!device_bytes || !n_states) return fail(BGK_E_INVALID, "null output");
+  bgk::adj_info(h, device_bytes, n_states);
   return BGK_OK;
 }
 

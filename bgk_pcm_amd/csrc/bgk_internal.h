@@ -117,8 +117,9 @@ hipError_t launch_gather(bgk_handle* h, const double* plane, int n, const int32_
 int reduce_partial_count(const bgk_handle* h);
 
 // ---- adjoint (bgk_adjoint.hip), FP64 moment handles
-hipError_t adj_setup(bgk_handle* h, int nsteps, int seg, int np, const int32_t* pi,
-                     const int32_t* pj);
+hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg_inner, int np,
+                     const int32_t* pi, const int32_t* pj);
+void adj_info(const bgk_handle* h, int64_t* bytes, int32_t* n_states);
 hipError_t adj_forward(bgk_handle* h, double* probe_T_host);
 hipError_t adj_backward(bgk_handle* h, const double* seeds_host, double grad[3]);
 void adj_free(bgk_handle* h);

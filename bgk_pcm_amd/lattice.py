@@ -96,6 +96,9 @@ SIGNATURES = {
     "bgk_synchronize": [_H],
     "bgk_get_info": [_H, ctypes.POINTER(BgkInfoC)],
     "bgk_adj_setup": [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _I32P, _I32P],
+    "bgk_adj_setup_levels": [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                             _I32P, _I32P],
+    "bgk_adj_info": [_H, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)],
     "bgk_adj_forward": [_H, _D],
     "bgk_adj_backward": [_H, _D, _D],
 }
@@ -159,12 +162,21 @@ class Lattice:
 
     # ---- adjoint (FP64 moment handles): probe-temperature record and its
     # gradient with respect to (stefan_number, k_ratio, t_hot)
-    def adj_setup(self, nsteps: int, seg_len: int, pi: Sequence[int], pj: Sequence[int]) -> None:
+    def adj_setup(self, nsteps: int, seg_len: int, pi: Sequence[int], pj: Sequence[int],
+                  seg_outer: int = 0) -> None:
+        """Checkpoints every seg_len steps; with seg_outer > 0 two levels
+        (every seg_outer steps, then every seg_len steps of one outer segment)."""
         self._adj_n, self._adj_np = int(nsteps), len(pi)
         ai = np.ascontiguousarray(pi, dtype=np.int32)
         aj = np.ascontiguousarray(pj, dtype=np.int32)
-        _check(self._lib.bgk_adj_setup(self._h, nsteps, seg_len, len(pi),
-                                       ai.ctypes.data_as(_I32P), aj.ctypes.data_as(_I32P)))
+        _check(self._lib.bgk_adj_setup_levels(self._h, nsteps, seg_outer, seg_len, len(pi),
+                                              ai.ctypes.data_as(_I32P), aj.ctypes.data_as(_I32P)))
+
+    def adj_info(self) -> tuple[int, int]:
+        """(device bytes of the adjoint's checkpoints and state, stored lattice states)."""
+        b, n = ctypes.c_int64(), ctypes.c_int32()
+        _check(self._lib.bgk_adj_info(self._h, ctypes.byref(b), ctypes.byref(n)))
+        return int(b.value), int(n.value)
 
     def adj_forward(self) -> np.ndarray:
         """Run the recorded forward; returns T at the probes, (nsteps, n_probes)."""

@@ -58,7 +58,7 @@ enum {
                        populations are not addressable (download/upload of
                        f, g, fp, gp, rho2..T2 and all uploads: unsupported) */
   BGK_FP64_MOM = 3  /* the same moment storage with every moment in FP64: the
-                       forward model of the adjoint (bgk_adj_*)              */
+                       exact forward model of the adjoint (bgk_adj_*)        */
 };
 
 /* rounding policy of the FP64 arithmetic (bgk_params.rounding) */
@@ -234,13 +234,24 @@ int bgk_gather(bgk_handle* h, int32_t field, int32_t n, const int32_t* i,
 int bgk_synchronize(bgk_handle* h);
 
 /* ---- Adjoint of the time step (no reference counterpart: the reference
- * has no derivatives).  Differentiates the FP64 moment step (precision
- * BGK_FP64_MOM, no-slip cavity) for a loss that depends on the averaged
- * temperature T (STEP_7/moments.f90:11-25) at n_probes interior nodes
- * (pi[p], pj[p]) after each of nsteps steps.  The state is checkpointed
- * every seg_len steps and recomputed segment by segment.
+ * has no derivatives).  Differentiates the moment step (precision
+ * BGK_FP64_MOM, or BGK_FP32_MOM: then the adjoint of the step evaluated at
+ * the FP32-stored states, the storage rounding taken as the identity;
+ * no-slip cavity) for a loss that depends on the averaged temperature T
+ * (STEP_7/moments.f90:11-25) at n_probes interior nodes (pi[p], pj[p]) after
+ * each of nsteps steps.  The state is checkpointed every seg_len steps and
+ * recomputed segment by segment.
  *
  * bgk_adj_setup     allocates the checkpoints and adjoint state.
+ * bgk_adj_setup_levels  the same with two levels of checkpoints: every
+ *                   seg_outer steps during the forward run, every seg_inner
+ *                   steps of one outer segment while it is recomputed, and
+ *                   every step of one inner segment: nsteps/seg_outer +
+ *                   seg_outer/seg_inner + seg_inner + 1 states instead of
+ *                   nsteps/seg_inner + seg_inner + 1, for one more forward
+ *                   recomputation.  seg_outer = 0: one level (bgk_adj_setup).
+ * bgk_adj_info      device bytes of the adjoint's checkpoints and state, and
+ *                   the number of stored lattice states among them.
  * bgk_adj_forward   runs nsteps steps from the current state (the
  *                   initialization, for the t_hot gradient to include the
  *                   initial liquid temperature), writing the probe record
@@ -250,6 +261,9 @@ int bgk_synchronize(bgk_handle* h);
  *                   (k_ratio and t_hot: bgk_ext). */
 int bgk_adj_setup(bgk_handle* h, int32_t nsteps, int32_t seg_len, int32_t n_probes,
                   const int32_t* pi, const int32_t* pj);
+int bgk_adj_setup_levels(bgk_handle* h, int32_t nsteps, int32_t seg_outer, int32_t seg_inner,
+                         int32_t n_probes, const int32_t* pi, const int32_t* pj);
+int bgk_adj_info(bgk_handle* h, int64_t* device_bytes, int32_t* n_states);
 int bgk_adj_forward(bgk_handle* h, double* probe_T);
 int bgk_adj_backward(bgk_handle* h, const double* seeds, double grad[3]);
 

@@ -17,7 +17,7 @@ import torch
 
 from bgk_pcm_amd.adjoint import probe_temperature, seeded_probes
 from bgk_pcm_amd.lattice import BgkError, Lattice
-from bgk_pcm_amd.params import FP32_MOM, FP64_MOM, ROUND_CLEAN, Params
+from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, FP64_MOM, ROUND_CLEAN, Params
 from torch_ref import probe_record
 
 pytestmark = pytest.mark.gpu
@@ -108,8 +108,8 @@ def test_autograd_function_gives_the_adjoint_gradient():
     np.testing.assert_allclose(theta.grad.numpy(), th.grad.numpy(), rtol=1e-7)
 
 
-def test_adjoint_needs_an_fp64_moment_handle():
-    with Lattice(Params(nx=8, ny=8, precision=FP32_MOM)) as lat:
+def test_adjoint_needs_a_moment_handle():
+    with Lattice(Params(nx=8, ny=8, precision=FP32_DEV)) as lat:
         with pytest.raises(BgkError) as e:
             lat.adj_setup(10, 5, [2], [3])
         assert e.value.code == -4
@@ -117,6 +117,58 @@ def test_adjoint_needs_an_fp64_moment_handle():
         with pytest.raises(BgkError) as e:
             lat.adj_setup(10, 5, [0], [3])
         assert e.value.code == -1
+        with pytest.raises(BgkError) as e:  # outer segment shorter than the inner one
+            lat.adj_setup(10, 5, [2], [3], seg_outer=3)
+        assert e.value.code == -1
+    with Lattice(Params(nx=8, ny=8, precision=FP32_MOM)) as lat:
+        lat.adj_setup(10, 5, [2], [3])
+
+
+def adjoint_gradient(p, prec, nsteps, seg, seg_outer, pi, pj, target, theta=(0.1, 1.5, 283.5)):
+    st, kr, th = theta
+    with Lattice(p.replace(precision=prec, stefan_number=st), initialize=False, t_hot=th,
+                 k_ratio=kr) as lat:
+        lat.initialization()
+        lat.adj_setup(nsteps, seg, pi, pj, seg_outer=seg_outer)
+        info = lat.adj_info()
+        rec = lat.adj_forward()
+        return lat.adj_backward(2.0 * (rec - target) / rec.size), rec, info
+
+
+@pytest.mark.parametrize("prec", [FP64_MOM, FP32_MOM])
+@pytest.mark.parametrize("tau_f", [1.0, 0.8])
+def test_two_level_checkpoints_give_the_same_gradient(prec, tau_f):
+    """Two levels recompute the same states: the gradient is bit-identical to
+    the one-level run, with nsteps/seg_outer + seg_outer/seg + seg + 1 - 1
+    states stored (the first inner checkpoint is the outer one)."""
+    p = Params(nx=40, ny=32, length=40.0, grav=2e-3, rounding=ROUND_CLEAN, tau_f=tau_f)
+    nsteps = 230
+    pi, pj = seeded_probes(p.nx, p.ny, 6, seed=5)
+    target = gpu_record(p, (0.12, 1.2, 283.2), pi, pj, nsteps, 10)
+    g1, r1, i1 = adjoint_gradient(p, prec, nsteps, 7, 0, pi, pj, target)
+    g2, r2, i2 = adjoint_gradient(p, prec, nsteps, 7, 40, pi, pj, target)
+    print("one level", g1, i1, "two levels", g2, i2, flush=True)
+    assert np.array_equal(r1, r2)
+    assert np.array_equal(g1, g2)
+    assert i1[1] == (nsteps + 6) // 7 + 7 + 1
+    assert i2[1] == (nsteps + 39) // 40 + (40 + 6) // 7 - 1 + 7 + 1
+    assert i2[0] < i1[0]
+
+
+def test_fp32_moment_adjoint_matches_the_fp64_adjoint():
+    """The adjoint over FP32-stored states against the FP64 one on the same
+    problem.  Tolerance 1e-4 relative per component (the gradient bar of the
+    FD check); the FP32 storage's own forward error is ~1e-6."""
+    p = Params(nx=96, ny=96, length=96.0, grav=1e-3, rounding=ROUND_CLEAN)
+    nsteps = 600
+    pi, pj = seeded_probes(p.nx, p.ny, 16, seed=3)
+    target = gpu_record(p, (0.12, 1.2, 283.2), pi, pj, nsteps, 50)
+    g64, r64, _ = adjoint_gradient(p, FP64_MOM, nsteps, 25, 150, pi, pj, target)
+    g32, r32, _ = adjoint_gradient(p, FP32_MOM, nsteps, 25, 150, pi, pj, target)
+    rel = np.abs(g32 - g64) / np.abs(g64)
+    print("FP64 adjoint", g64, "FP32 adjoint", g32, "relative difference", rel,
+          "record difference", np.abs(r32 - r64).max(), flush=True)
+    assert np.all(rel <= 1e-4), rel
 
 
 def test_adjoint_with_cold_wall_and_noise_matches_central_differences():
