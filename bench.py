@@ -326,15 +326,16 @@ def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: 
             g, tf, tb, adj_bytes, nstates, hb = gradient(b, pi, pj, target, theta0)
             out[(size, prec)] = (g, tf, tb, adj_bytes, nstates, hb)
     g, tf, tb, adj_bytes, nstates, hb = out[(n, FP32_MOM)]
-    # plain forward steps of the same handle type, for the cost ratio
+    # plain forward steps of the same handle type over the same steps (the
+    # step gets slower as the melting front widens), for the cost ratio
     progress(f"adjoint {n}^2: plain forward steps")
-    with Lattice(cavity(n).replace(precision=FP32_MOM, rounding=ROUND_CLEAN)) as lat:
-        lat.step(20)
+    with Lattice(cavity(n).replace(precision=FP32_MOM, rounding=ROUND_CLEAN,
+                                   stefan_number=0.1)) as lat:
         lat.synchronize()
         t0 = time.perf_counter()
-        lat.step(500)
+        lat.step(nsteps)
         lat.synchronize()
-        t_plain = (time.perf_counter() - t0) / 500
+        t_plain = (time.perf_counter() - t0) / nsteps
     g32, g64 = out[(check_n, FP32_MOM)][0], out[(check_n, FP64_MOM)][0]
     return {"workload": f"{n}x{n} cavity, FP32 moments, Ra 1e7, St 0.1, {nsteps} steps, "
                         f"checkpoints every {seg_outer} and {seg} steps, {n_probes} probes",

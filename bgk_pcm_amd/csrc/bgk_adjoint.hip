@@ -38,8 +38,6 @@ namespace {
 
 using namespace mom;
 
-constexpr int TX = 64, TY = 4;
-
 // one stored state: moment planes, phi, east-wall column
 struct StatePtrs {
   double2* a;
@@ -72,20 +70,60 @@ struct AdjArgs {
   int interf;
 };
 
-// Adjoint of population k as pulled by node q, from that node's adjoint of
-// its pulled moments (PullSums is linear in the populations).
-__device__ __forceinline__ double lam_f(const AdjPlanes& L, int k, int64_t q) {
-  double r = L.v[L_RHO][q];
-  if (CX[k] != 0) r += (CX[k] > 0 ? L.v[L_JX][q] : -L.v[L_JX][q]) + L.v[L_FXX][q];
-  if (CY[k] != 0) r += (CY[k] > 0 ? L.v[L_JY][q] : -L.v[L_JY][q]) + L.v[L_FYY][q];
-  if (CX[k] * CY[k] != 0) r += CX[k] * CY[k] > 0 ? L.v[L_FXY][q] : -L.v[L_FXY][q];
+// The workgroup's window in LDS: a tile of BX x BY nodes and a one-node
+// halo, with the adjoint planes of step n+1 (those read at neighbours: not
+// L_PC, and with omega_f = 1 (EQF) not the second moments of f, whose
+// adjoints are (1 - omega_f) times finite values, exact zeros, and are
+// neither stored nor read) and the state before step n (moments and phi).
+// Every read of a node's update lies in its 3 x 3 neighbourhood, so each
+// value comes from HBM once per tile instead of from L2 for every neighbour.
+constexpr int BX = 64, BY = 4, WX = BX + 2, WY = BY + 2;
+
+template <bool EQF>
+struct LSlots {
+  static constexpr int N = EQF ? 9 : 12;
+  // plane staged in LDS slot s
+  static constexpr int plane[12] = {L_RHO, L_JX, L_JY, L_DT, L_GXX, L_GYY,
+                                    L_GXY, L_KP, L_KM, L_FXX, L_FYY, L_FXY};
+  enum { RHO, JX, JY, DT, GXX, GYY, GXY, KP, KM, FXX, FYY, FXY };
+};
+
+template <typename R, bool EQF>
+struct Win {
+  double lam[LSlots<EQF>::N][WY][WX];
+  double2 a[WY][WX];
+  typename Vec<R>::V4 b[WY][WX];
+  typename CType<R, EQF>::T c[WY][WX];
+  double phi[WY][WX];
+};
+
+// Adjoint of population k as pulled by the node at window (x, y), from that
+// node's adjoint of its pulled moments (PullSums is linear in the
+// populations).
+template <typename R, bool EQF>
+__device__ __forceinline__ double lam_f(const Win<R, EQF>& W, int k, int x, int y) {
+  using S = LSlots<EQF>;
+  double r = W.lam[S::RHO][y][x];
+  if constexpr (EQF) {
+    if (CX[k] != 0) r += CX[k] > 0 ? W.lam[S::JX][y][x] : -W.lam[S::JX][y][x];
+    if (CY[k] != 0) r += CY[k] > 0 ? W.lam[S::JY][y][x] : -W.lam[S::JY][y][x];
+  } else {
+    if (CX[k] != 0)
+      r += (CX[k] > 0 ? W.lam[S::JX][y][x] : -W.lam[S::JX][y][x]) + W.lam[S::FXX][y][x];
+    if (CY[k] != 0)
+      r += (CY[k] > 0 ? W.lam[S::JY][y][x] : -W.lam[S::JY][y][x]) + W.lam[S::FYY][y][x];
+    if (CX[k] * CY[k] != 0)
+      r += CX[k] * CY[k] > 0 ? W.lam[S::FXY][y][x] : -W.lam[S::FXY][y][x];
+  }
   return r;
 }
-__device__ __forceinline__ double lam_g(const AdjPlanes& L, int k, int64_t q) {
-  double r = L.v[L_DT][q];
-  if (CX[k] != 0) r += L.v[L_GXX][q];
-  if (CY[k] != 0) r += L.v[L_GYY][q];
-  if (CX[k] * CY[k] != 0) r += CX[k] * CY[k] > 0 ? L.v[L_GXY][q] : -L.v[L_GXY][q];
+template <typename R, bool EQF>
+__device__ __forceinline__ double lam_g(const Win<R, EQF>& W, int k, int x, int y) {
+  using S = LSlots<EQF>;
+  double r = W.lam[S::DT][y][x];
+  if (CX[k] != 0) r += W.lam[S::GXX][y][x];
+  if (CY[k] != 0) r += W.lam[S::GYY][y][x];
+  if (CX[k] * CY[k] != 0) r += CX[k] * CY[k] > 0 ? W.lam[S::GXY][y][x] : -W.lam[S::GXY][y][x];
   return r;
 }
 
@@ -156,26 +194,26 @@ __device__ __forceinline__ double ghost_dT(int k, double u, double v) {
   return kW[k] * (1.0 + eq_poly(k, u, v));
 }
 
-// Pulled sums and taps of step n at node (i, j) from the state before the
-// step, with the boundary folding of the forward kernels (mom_edge_nodes).
+// Pulled sums and taps of step n at node (i, j), window (ox, oy), from the
+// state before the step, with the boundary folding of the forward kernels
+// (mom_edge_nodes).
 template <typename R, bool EQF>
-__device__ __forceinline__ void pull_step(const AdjArgs& a, int i, int j, PullSums* S,
-                                          double tap[9]) {
+__device__ __forceinline__ void pull_step(const AdjArgs& a, const Win<R, EQF>& W, int i, int j,
+                                          int ox, int oy, PullSums* S, double tap[9]) {
   const int nx = a.gm.nx, ny = a.gm.ny;
   const int64_t P = a.gm.pitch;
-  const int64_t q = (int64_t)j * P + i;
-  const StatePtrs& s = a.sprev;
   pull_zero(*S);
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     const int si = i - CX[k], sj = j - CY[k];
     const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
+    const int wx = ox - CX[k], wy = oy + (sjc - j);
     double fk = 0.0, gk;
-    const int64_t h = (int64_t)sjc * P + si;
     if (si >= 1 && si <= nx) {
-      rebuild<R, EQF>(k, s.a[h], b_at<R, EQF>(s.b, h), c_at<R, EQF>(s.c, h), &fk, &gk);
+      rebuild<R, EQF>(k, W.a[wy][wx], W.b[wy][wx], W.c[wy][wx], &fk, &gk);
     } else {
-      gk = ghost_gdev(k, si == 0 ? a.c.tin : east_T(a.c, s.te, sjc), a.uh[h], a.vh[h]);
+      const int64_t h = (int64_t)sjc * P + si;
+      gk = ghost_gdev(k, si == 0 ? a.c.tin : east_T(a.c, a.sprev.te, sjc), a.uh[h], a.vh[h]);
     }
     if (si < 1 || si > nx || sj < 1 || sj > ny) {
       if (k == 6 && si == nx + 1 && sj == 0)
@@ -183,23 +221,20 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, int i, int j, PullSu
       else if (k == 7 && si == nx + 1 && sj == ny + 1)
         fk = a.c.frozen7;
       else
-        fk = rebuild_f<R, EQF>(OPP[k], s.a[q], b_at<R, EQF>(s.b, q), c_at<R, EQF>(s.c, q));
+        fk = rebuild_f<R, EQF>(OPP[k], W.a[oy][ox], W.b[oy][ox], W.c[oy][ox]);
     }
     pull_add(*S, k, fk, gk);
   }
-  const int64_t rn = (int64_t)(j < ny ? j + 1 : ny) * P;
-  const int64_t rs = (int64_t)(j > 1 ? j - 1 : 1) * P;
-  const int64_t r0 = (int64_t)j * P;
-  const double* ph = s.phi;
-  tap[0] = ph[q];
-  tap[1] = ph[r0 + i + 1];
-  tap[2] = ph[rn + i];
-  tap[3] = ph[r0 + i - 1];
-  tap[4] = ph[rs + i];
-  tap[5] = ph[rn + i + 1];
-  tap[6] = ph[rn + i - 1];
-  tap[7] = ph[rs + i - 1];
-  tap[8] = ph[rs + i + 1];
+  const int yn = oy + (j < ny ? 1 : 0), ys = oy - (j > 1 ? 1 : 0);
+  tap[0] = W.phi[oy][ox];
+  tap[1] = W.phi[oy][ox + 1];
+  tap[2] = W.phi[yn][ox];
+  tap[3] = W.phi[oy][ox - 1];
+  tap[4] = W.phi[ys][ox];
+  tap[5] = W.phi[yn][ox + 1];
+  tap[6] = W.phi[yn][ox - 1];
+  tap[7] = W.phi[ys][ox - 1];
+  tap[8] = W.phi[ys][ox + 1];
 }
 
 // Reverse of core_forward (bgk_mom.h) and of the state it stores.  lo: the
@@ -289,11 +324,34 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S
 }
 
 template <typename R, bool EQF>
-__global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
+__global__ __launch_bounds__(BX* BY) void k_adjoint(AdjArgs a) {
   const int nx = a.gm.nx, ny = a.gm.ny;
   const int64_t P = a.gm.pitch;
-  const int i = blockIdx.x * TX + threadIdx.x + 1;
-  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  const int i0 = blockIdx.x * BX + 1, j0 = blockIdx.y * BY + 1;
+  const int ox = threadIdx.x + 1, oy = threadIdx.y + 1;  // this node in the window
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  // ---- the window: rows j0-1 .. j0+BY, columns i0-1 .. i0+BX, clamped to
+  // the planes' extent (clamped entries are never read: every read below is
+  // guarded as in the global form)
+  __shared__ Win<R, EQF> W;
+  {
+    using S = LSlots<EQF>;
+    for (int e = threadIdx.y * BX + threadIdx.x; e < WX * WY; e += BX * BY) {
+      const int ex = e % WX, ey = e / WX;
+      int xi = i0 - 1 + ex, xj = j0 - 1 + ey;
+      xi = xi > nx + 1 ? nx + 1 : xi;
+      xj = xj > ny + 1 ? ny + 1 : xj;
+      const int64_t x = (int64_t)xj * P + xi;
+      if (a.has_next)
+#pragma unroll
+        for (int sl = 0; sl < S::N; ++sl) W.lam[sl][ey][ex] = a.next.v[S::plane[sl]][x];
+      W.a[ey][ex] = a.sprev.a[x];
+      W.b[ey][ex] = b_at<R, EQF>(a.sprev.b, x);
+      W.c[ey][ex] = c_at<R, EQF>(a.sprev.c, x);
+      W.phi[ey][ex] = a.sprev.phi[x];
+    }
+  }
+  __syncthreads();
   double g[3] = {0.0, 0.0, 0.0};
   if (i <= nx && j <= ny) {
     const int64_t q = (int64_t)j * P + i;
@@ -302,27 +360,26 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
     double lphi = 0.0;
     double lTrep = 0.0;
     if (a.has_next) {
-      const AdjPlanes& L = a.next;
+      using S = LSlots<EQF>;
       // ---- reverse of step n+1's pulls from this node
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
         const int xi = i + CX[k], xj = j + CY[k];
-        if (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny) {
-          const int64_t x = (int64_t)xj * P + xi;
-          rebuild_adjoint(k, m, lam_f(L, k, x), lam_g(L, k, x), &lm);
-        }
+        if (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny)
+          rebuild_adjoint(k, m, lam_f(W, k, ox + CX[k], oy + CY[k]),
+                          lam_g(W, k, ox + CX[k], oy + CY[k]), &lm);
         // this node's own pull of direction k from a ghost: bounce-back of
         // its opposite population (not for the two frozen corner ghosts)
         const int si = i - CX[k], sj = j - CY[k];
         if (si < 1 || si > nx || sj < 1 || sj > ny) {
           const bool frozen = (k == 6 && si == nx + 1 && sj == 0) ||
                               (k == 7 && si == nx + 1 && sj == ny + 1);
-          if (!frozen) rebuild_adjoint(OPP[k], m, lam_f(L, k, q), 0.0, &lm);
+          if (!frozen) rebuild_adjoint(OPP[k], m, lam_f(W, k, ox, oy), 0.0, &lm);
           // west ghost: equilibrium at the hot-wall temperature
           if (si == 0) {
             const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
             const int64_t h = (int64_t)sjc * P;
-            g[2] += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(L, k, q);
+            g[2] += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(W, k, ox, oy);
           }
         }
       }
@@ -332,10 +389,7 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
         for (int k = 0; k < 9; ++k)
           if (CY[k] > 0) {
             const int xi = i + CX[k];
-            if (xi >= 1 && xi <= nx) {
-              const int64_t x = (int64_t)1 * P + xi;
-              rebuild_adjoint(k, m, 0.0, lam_g(L, k, x), &lm);
-            }
+            if (xi >= 1 && xi <= nx) rebuild_adjoint(k, m, 0.0, lam_g(W, k, ox + CX[k], oy), &lm);
           }
       }
       if (j == ny) {
@@ -343,10 +397,7 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
         for (int k = 0; k < 9; ++k)
           if (CY[k] < 0) {
             const int xi = i + CX[k];
-            if (xi >= 1 && xi <= nx) {
-              const int64_t x = (int64_t)ny * P + xi;
-              rebuild_adjoint(k, m, 0.0, lam_g(L, k, x), &lm);
-            }
+            if (xi >= 1 && xi <= nx) rebuild_adjoint(k, m, 0.0, lam_g(W, k, ox + CX[k], oy), &lm);
           }
       }
       // east ghost: equilibrium at this node's averaged T of step n (not
@@ -354,7 +405,6 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
       if (i == nx && !a.c.east_cold) {
         for (int xj = j - 1; xj <= j + 1; ++xj) {
           if (xj < 1 || xj > ny) continue;
-          const int64_t x = (int64_t)xj * P + nx;
 #pragma unroll
           for (int k = 0; k < 9; ++k) {
             if (CX[k] >= 0) continue;  // directions 3, 6, 7 pull from column Nx+1
@@ -362,7 +412,7 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
             const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
             if (sjc != j) continue;
             const int64_t h = (int64_t)j * P + nx + 1;
-            lTrep += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(L, k, x);
+            lTrep += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(W, k, ox, oy + (xj - j));
           }
         }
       }
@@ -375,21 +425,19 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
       for (int t = 0; t < 9; ++t) {
         // nodes x whose tap t reads this node: x = y - d, and in rows 1 / Ny
         // also those whose tap t reads the ghost row copied from this node
-        int xi = i - DX[t], xj = j - DY[t];
+        const int xi = i - DX[t], xj = j - DY[t];
         if (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny) {
-          const int64_t x = (int64_t)xj * P + xi;
-          lphi += aa * L.v[L_KP][x] - bb * L.v[L_KM][x];
+          const int wx = ox - DX[t], wy = oy - DY[t];
+          lphi += aa * W.lam[S::KP][wy][wx] - bb * W.lam[S::KM][wy][wx];
         }
         if ((j == 1 && DY[t] < 0) || (j == ny && DY[t] > 0)) {
-          xi = i - DX[t];
-          xj = j;
           if (xi >= 1 && xi <= nx) {
-            const int64_t x = (int64_t)xj * P + xi;
-            lphi += aa * L.v[L_KP][x] - bb * L.v[L_KM][x];
+            const int wx = ox - DX[t];
+            lphi += aa * W.lam[S::KP][oy][wx] - bb * W.lam[S::KM][oy][wx];
           }
         }
       }
-      lphi += L.v[L_PC][q];
+      lphi += a.next.v[L_PC][q];
     }
     if (a.core) {
       // ---- loss seed at a probe node (averaged T after step n)
@@ -402,7 +450,7 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
       // ---- reverse of step n at this node
       PullSums S;
       double tap[9];
-      pull_step<R, EQF>(a, i, j, &S, tap);
+      pull_step<R, EQF>(a, W, i, j, ox, oy, &S, tap);
       const Core o = core_forward(a.c, S, tap);
       PullSums lS;
       double lkp, lkm, lpc;
@@ -411,9 +459,11 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
       v[L_RHO][q] = lS.rho;
       v[L_JX][q] = lS.jx;
       v[L_JY][q] = lS.jy;
-      v[L_FXX][q] = lS.fxx;
-      v[L_FYY][q] = lS.fyy;
-      v[L_FXY][q] = lS.fxy;
+      if constexpr (!EQF) {
+        v[L_FXX][q] = lS.fxx;
+        v[L_FYY][q] = lS.fyy;
+        v[L_FXY][q] = lS.fxy;
+      }
       v[L_DT][q] = lS.dT;
       v[L_GXX][q] = lS.gxx;
       v[L_GYY][q] = lS.gyy;
@@ -427,11 +477,11 @@ __global__ __launch_bounds__(TX* TY) void k_adjoint(AdjArgs a) {
     }
   }
   // ---- parameter adjoints: workgroup sum into this workgroup's own slot
-  __shared__ double red[3][TX * TY];
-  const int t = threadIdx.y * TX + threadIdx.x;
+  __shared__ double red[3][BX * BY];
+  const int t = threadIdx.y * BX + threadIdx.x;
   for (int c = 0; c < 3; ++c) red[c][t] = g[c];
   __syncthreads();
-  for (int w = TX * TY / 2; w > 0; w >>= 1) {
+  for (int w = BX * BY / 2; w > 0; w >>= 1) {
     if (t < w)
       for (int c = 0; c < 3; ++c) red[c][t] += red[c][t + w];
     __syncthreads();
@@ -461,7 +511,8 @@ __global__ void k_adj_sum(const double* partial, int n, double* out) {
 }
 
 // probe index plane: -1 everywhere (k_probe_clear), then p at probe p's
-// node (k_probe_set, a second launch on the same stream)
+// node and 0 in column 0 of its row (k_probe_set, a second launch on the
+// same stream): the forward kernels read the row flag once per row
 __global__ void k_probe_clear(int* idx, int64_t n) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) idx[q] = -1;
@@ -469,7 +520,10 @@ __global__ void k_probe_clear(int* idx, int64_t n) {
 
 __global__ void k_probe_set(int* idx, const int* pij, int np, int64_t pitch) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < np) idx[(int64_t)pij[2 * p + 1] * pitch + pij[2 * p]] = p;
+  if (p < np) {
+    idx[(int64_t)pij[2 * p + 1] * pitch + pij[2 * p]] = p;
+    idx[(int64_t)pij[2 * p + 1] * pitch] = 0;
+  }
 }
 
 }  // namespace
@@ -521,14 +575,25 @@ struct Bump {
     off += n * sizeof(T);
     return p;
   }
+  // a lattice plane of elements of esize bytes placed as the handle places
+  // its own: element 1 (column 1 of row 0) on a 256-byte boundary, so that
+  // with the 64-element pitch every row's column 1 is (the kernels' row
+  // accesses then cover whole cache lines)
+  template <typename T>
+  T* plane(size_t n, size_t esize = sizeof(T)) {
+    off = (off + 255) / 256 * 256 + 256 - esize;
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += n * esize;
+    return p;
+  }
 };
 
 void carve_state(Bump& b, const bgk_handle* h, StatePtrs* s) {
   const size_t n = (size_t)h->plane_elems;
-  s->a = b.take<double2>(n);
-  s->b = b.take<char>(n * b_bytes(h));
-  s->c = b.take<char>(n * c_bytes(h));
-  s->phi = b.take<double>(n);
+  s->a = b.plane<double2>(n);
+  s->b = b.plane<char>(n, b_bytes(h));
+  s->c = b.plane<char>(n, c_bytes(h));
+  s->phi = b.plane<double>(n);
   s->te = b.take<double>((size_t)h->ny + 2);
 }
 
@@ -542,12 +607,15 @@ size_t carve_ctx(bgk_handle* h, AdjCtx* a, char* base) {
   for (auto& s : a->ckpt) carve_state(b, h, &s);
   for (auto& s : a->ckpt2) carve_state(b, h, &s);
   for (int l = 0; l < 2; ++l)
-    for (int v = 0; v < 13; ++v) a->lam[l].v[v] = b.take<double>(n);
-  a->probe_idx = b.take<int>(n);
+    for (int v = 0; v < 13; ++v) {
+      const bool zero = h->mom_eqf && (v == L_FXX || v == L_FYY || v == L_FXY);
+      a->lam[l].v[v] = zero ? nullptr : b.plane<double>(n);
+    }
+  a->probe_idx = b.plane<int>(n);
   a->probe_ij = b.take<int>(2 * (size_t)std::max(a->np, 1));
   a->probe_T = b.take<double>((size_t)a->nsteps * std::max(a->np, 1));
   a->seeds = b.take<double>((size_t)a->nsteps * std::max(a->np, 1));
-  for (int m = 0; m < 4; ++m) a->macro[m] = b.take<double>(n);
+  for (int m = 0; m < 4; ++m) a->macro[m] = b.plane<double>(n);
   a->partial = b.take<double>(3 * (size_t)a->nblk);
   a->out3 = b.take<double>(3);
   return b.off + 256;
@@ -586,16 +654,16 @@ hipError_t launch_adjoint(bgk_handle* h, AdjCtx* x, const StatePtrs& sn, const S
   a.c = h->c;
   a.gm = h->geom;
   a.interf = h->nx / 40;
-  dim3 grd((unsigned)((h->nx + TX - 1) / TX), (unsigned)((h->ny + TY - 1) / TY));
+  dim3 grd((unsigned)((h->nx + BX - 1) / BX), (unsigned)((h->ny + BY - 1) / BY));
   const bool f64 = h->prm.precision == BGK_FP64_MOM;
   if (h->mom_eqf && f64)
-    hipLaunchKernelGGL((k_adjoint<double, true>), grd, dim3(TX, TY), 0, h->stream, a);
+    hipLaunchKernelGGL((k_adjoint<double, true>), grd, dim3(BX, BY), 0, h->stream, a);
   else if (h->mom_eqf)
-    hipLaunchKernelGGL((k_adjoint<float, true>), grd, dim3(TX, TY), 0, h->stream, a);
+    hipLaunchKernelGGL((k_adjoint<float, true>), grd, dim3(BX, BY), 0, h->stream, a);
   else if (f64)
-    hipLaunchKernelGGL((k_adjoint<double, false>), grd, dim3(TX, TY), 0, h->stream, a);
+    hipLaunchKernelGGL((k_adjoint<double, false>), grd, dim3(BX, BY), 0, h->stream, a);
   else
-    hipLaunchKernelGGL((k_adjoint<float, false>), grd, dim3(TX, TY), 0, h->stream, a);
+    hipLaunchKernelGGL((k_adjoint<float, false>), grd, dim3(BX, BY), 0, h->stream, a);
   return hipGetLastError();
 }
 
@@ -638,7 +706,7 @@ hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg, int np,
   x->nckpt = (nsteps + lvl1 - 1) / lvl1;
   // (the first checkpoint of an outer segment is the outer checkpoint)
   x->nckpt2 = seg_outer > 0 ? (std::min(seg_outer, nsteps) + seg - 1) / seg - 1 : 0;
-  x->nblk = (int)(((h->nx + TX - 1) / TX) * ((h->ny + TY - 1) / TY));
+  x->nblk = (int)(((h->nx + BX - 1) / BX) * ((h->ny + BY - 1) / BY));
   x->bytes = carve_ctx(h, x, nullptr);
   hipError_t e = hipMalloc(&x->pool, x->bytes);
   if (e != hipSuccess) {

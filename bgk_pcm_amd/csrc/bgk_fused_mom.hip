@@ -155,6 +155,14 @@ __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
     put(L, slot(jj), t + 1, load_node<R, EQF>(in, pa, gm, i, clampj(jj)));
     if (t < 2) put(L, slot(jj), hslot, load_node<R, EQF>(in, pa, gm, hcol, clampj(jj)));
   }
+  // rows j0 .. j0+63 of the strip that hold a probe (the flag in column 0
+  // of the probe index plane): one load per lane here instead of a load and
+  // its wait in every row
+  uint64_t pmask = 0;
+  if (probe_out) {
+    const int jr = min(j0 + (t & 63), j1);
+    pmask = __ballot(probe_idx[(int64_t)jr * P] >= 0);
+  }
   __syncthreads();
 
   for (int j = j0; j <= j1; ++j) {
@@ -182,7 +190,10 @@ __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
       const double tap[9] = {L.phi[rC][s],     L.phi[rC][s + 1], L.phi[rN][s],
                              L.phi[rC][s - 1], L.phi[rS][s],     L.phi[rN][s + 1],
                              L.phi[rN][s - 1], L.phi[rS][s - 1], L.phi[rS][s + 1]};
-      const NodeOut o = update_node<R, EQF>(c, S, tap, out, pb, q, probe_idx, probe_out);
+      const bool prow = probe_out && (j - j0 < 64 ? ((pmask >> (j - j0)) & 1) != 0
+                                                   : probe_idx[(int64_t)j * P] >= 0);
+      double* const po = prow ? probe_out : nullptr;
+      const NodeOut o = update_node<R, EQF>(c, S, tap, out, pb, q, probe_idx, po);
       if (write_all) {
         m.rho[q] = o.rho;
         m.T[q] = o.Tav;

@@ -312,7 +312,8 @@ struct NodeOut {
 // One launch of the moment-storage step: state and phi in and out, the
 // east-wall temperature column of the previous and of this step, an
 // optional probe record (T averaged as moments.f90 at the nodes whose
-// probe_idx is >= 0, written to probe_out[probe_idx]) and optional
+// probe_idx is >= 0, written to probe_out[probe_idx]; probe_idx in column 0
+// of a row is >= 0 when the row holds a probe) and optional
 // macroscopic planes rho, u, v, T to write instead of the handle's (their
 // halo columns of u, v must hold the ghost velocities).
 struct MomStep {
