@@ -277,7 +277,7 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S
   const double s = o.irho * c.inv_omega_f;
   const double fx = -(o.u2 * (1.0 - o.phi) * 0.5);
   const double gb = c.grav * c.beta;
-  const double fy0 = o.rho * gb * o.T * (1.0 + o.phi * 0.5);
+  const double fy0 = o.rho * gb * (o.T - c.t_ref) * (1.0 + o.phi * 0.5);
   const double fy = fy0 - o.v2 * (1.0 - o.phi) * 0.5;
   double lu2 = lu, lv2 = lv;
   const double lfx = s * lu, lfy = s * lv;
@@ -287,10 +287,10 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S
   lphi += o.v2 * 0.5 * lfy;
   lu2 += -(1.0 - o.phi) * 0.5 * lfx;
   lphi += o.u2 * 0.5 * lfx;
-  // fy0 = rho g beta T (1 + phi/2)
-  lrho += gb * o.T * (1.0 + o.phi * 0.5) * lfy;
+  // fy0 = rho g beta (T - t_ref) (1 + phi/2)
+  lrho += gb * (o.T - c.t_ref) * (1.0 + o.phi * 0.5) * lfy;
   lT += o.rho * gb * (1.0 + o.phi * 0.5) * lfy;
-  lphi += o.rho * gb * o.T * 0.5 * lfy;
+  lphi += o.rho * gb * (o.T - c.t_ref) * 0.5 * lfy;
   // stored T - T_crit = dT; T = T_crit + dT; dT = dT2 - dt_st (phi - pc)/2
   const double ldT = lT + lo.th;
   ldT2 += ldT;
@@ -305,7 +305,7 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S
   *lkp = rp * 0.125 * lkpr;
   *lkm = o.rm * 0.125 * lkmr;
   const double lrm = o.km_sum * 0.125 * lkmr - o.kp_sum * 0.125 * lkpr;
-  // rm = 1/(1 + exp(dT2 * 2/T_width))
+  // rm = 1/(1 + exp((dT2 + crit_shift) * 2/T_width))
   ldT2 += -o.rm * rp * c.two_over_tw * lrm;
   *lpc = lpcv;
   // u2 = jx irho, v2 = jy irho, irho = 1/rho

@@ -172,6 +172,8 @@ void set_constants(bgk_handle* h) {
   c.t_cold = h->ext.t_cold;
   c.noise_amp = h->ext.noise_amp * (p.t0 - p.t_in_sol);
   c.noise_seed = h->ext.noise_seed;
+  c.crit_shift = h->ext.t_crit > 0.0 ? 273.0 - h->ext.t_crit : 0.0;
+  c.t_ref = h->ext.t_ref;
   const double kvisc = c.cs2 * (p.tau_f - 0.5);
   c.u0 = p.re * kvisc / p.length;
   c.v0 = 0.0;
@@ -410,14 +412,17 @@ int bgk_set_extensions(bgk_handle* h, const bgk_ext* e) {
   if (rc) return rc;
   if (!e) return fail(BGK_E_INVALID, "null extensions");
   if (!(e->t_hot >= 0.0) || !(e->k_ratio > 0.0) || !(e->noise_amp >= 0.0) ||
-      (e->east_cold != 0 && e->east_cold != 1) || (e->east_cold && !(e->t_cold > 0.0)))
+      (e->east_cold != 0 && e->east_cold != 1) || (e->east_cold && !(e->t_cold > 0.0)) ||
+      !(e->t_crit >= 0.0) || !(e->t_ref >= 0.0))
     return fail(BGK_E_INVALID,
-                "t_hot=%g k_ratio=%g east_cold=%d t_cold=%g noise_amp=%g: need t_hot >= 0, "
-                "k_ratio > 0, east_cold 0/1 with t_cold > 0, noise_amp >= 0",
-                e->t_hot, e->k_ratio, e->east_cold, e->t_cold, e->noise_amp);
+                "t_hot=%g k_ratio=%g east_cold=%d t_cold=%g noise_amp=%g t_crit=%g t_ref=%g: "
+                "need t_hot >= 0, k_ratio > 0, east_cold 0/1 with t_cold > 0, noise_amp >= 0, "
+                "t_crit >= 0, t_ref >= 0",
+                e->t_hot, e->k_ratio, e->east_cold, e->t_cold, e->noise_amp, e->t_crit, e->t_ref);
   const bool mom = h->prm.precision == BGK_FP32_MOM || h->prm.precision == BGK_FP64_MOM;
   const bool off = (e->t_hot == 0.0 || e->t_hot == h->prm.t0) && e->k_ratio == 1.0 &&
-                   !e->east_cold && e->noise_amp == 0.0;
+                   !e->east_cold && e->noise_amp == 0.0 &&
+                   (e->t_crit == 0.0 || e->t_crit == 273.0) && e->t_ref == 0.0;
   if (!mom && !off)
     return fail(BGK_E_UNSUPPORTED, "physics extensions need a moment handle "
                                    "(precision BGK_FP32_MOM or BGK_FP64_MOM)");

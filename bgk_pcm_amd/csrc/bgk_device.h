@@ -114,6 +114,8 @@ struct Consts {
   double t_cold;
   double noise_amp;    // initial T noise amplitude in kelvin (0: none)
   uint64_t noise_seed;
+  double crit_shift;   // 273 - t_crit of the phase field (0: STEP_7)
+  double t_ref;        // buoyancy reference temperature (0: STEP_7's absolute T)
 };
 
 // Uniform [0,1) number of node q for the initial noise: SplitMix64 output

@@ -230,7 +230,9 @@ __device__ __forceinline__ void pull_add(PullSums& S, int k, double fk, double g
 // by exact zeros anyway, and a branch splits the node's code (~70 more
 // registers).
 //
-// Extension (off when k_ratio == 1): solid/liquid conductivity ratio through
+// Extensions: the phase field's T_crit (273 + crit_shift off) and a
+// buoyancy reference t_ref (0: STEP_7's absolute T; both exact when off).
+// Off when k_ratio == 1: solid/liquid conductivity ratio through
 // a phase-dependent thermal relaxation, alpha(phi) = alpha_l (fl + k_ratio
 // (1 - fl)), fl = (1 + phi)/2 the liquid fraction, tau_g = 1/2 + alpha/cs2.
 struct Core {
@@ -277,7 +279,7 @@ __device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
   // adjoint multiplies its rate terms by the same exact zeros).
   const bool react = !((kp == 0.0 || o.pc == 1.0) && (km == 0.0 || o.pc == -1.0));
   o.rm = 0.5;
-  if (react) o.rm = 1.0 / (1.0 + exp(o.dT2 * c.two_over_tw));
+  if (react) o.rm = 1.0 / (1.0 + exp((o.dT2 + c.crit_shift) * c.two_over_tw));
   const double rp = 1.0 - o.rm;
   const double kpr = (kp * 0.125) * rp;
   const double kmr = (km * 0.125) * o.rm;
@@ -285,7 +287,7 @@ __device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
 
   o.dT = o.dT2 - (c.dt_st * (o.phi - o.pc) * 0.5);
   o.T = kTcrit + o.dT;
-  double fy = o.rho * c.grav * c.beta * o.T * (1.0 + o.phi * 0.5);
+  double fy = o.rho * c.grav * c.beta * (o.T - c.t_ref) * (1.0 + o.phi * 0.5);
   const double fx = -(o.u2 * (1.0 - o.phi) * 0.5);
   fy = fy - o.v2 * (1.0 - o.phi) * 0.5;
   const double s = o.irho * c.inv_omega_f;

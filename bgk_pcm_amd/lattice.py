@@ -61,7 +61,8 @@ class BgkExtC(ctypes.Structure):
     _fields_ = [("t_hot", ctypes.c_double), ("k_ratio", ctypes.c_double),
                 ("east_cold", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("t_cold", ctypes.c_double), ("noise_amp", ctypes.c_double),
-                ("noise_seed", ctypes.c_uint64)]
+                ("noise_seed", ctypes.c_uint64), ("t_crit", ctypes.c_double),
+                ("t_ref", ctypes.c_double)]
 
 
 _lib = None
@@ -131,15 +132,16 @@ def _check(rc: int) -> None:
 class Lattice:
     """One lattice on the GPU; see the module docstring.
 
-    t_hot, k_ratio, t_cold, noise_amp, noise_seed: physics extensions of
-    bgk_set_extensions (hot-wall and initial liquid temperature;
-    solid/liquid conductivity ratio; east wall at t_cold when given; seeded
-    initial temperature noise), applied before the initialization; the
-    defaults give STEP_7."""
+    t_hot, k_ratio, t_cold, noise_amp, noise_seed, t_crit, t_ref: physics
+    extensions of bgk_set_extensions (hot-wall and initial liquid
+    temperature; solid/liquid conductivity ratio; east wall at t_cold when
+    given; seeded initial temperature noise; the phase field's critical
+    temperature; the buoyancy's reference temperature), applied before the
+    initialization; the defaults give STEP_7."""
 
     def __init__(self, params: Params, initialize: bool = True, t_hot: float = 0.0,
                  k_ratio: float = 1.0, t_cold: float | None = None, noise_amp: float = 0.0,
-                 noise_seed: int = 0):
+                 noise_seed: int = 0, t_crit: float = 0.0, t_ref: float = 0.0):
         self.params = params
         self.nx, self.ny = params.nx, params.ny
         self._lib = load_library()
@@ -148,16 +150,17 @@ class Lattice:
         _check(self._lib.bgk_create(ctypes.byref(self._c), ctypes.byref(h)))
         self._h = h
         self.it = 0
-        if t_hot != 0.0 or k_ratio != 1.0 or t_cold is not None or noise_amp != 0.0:
-            self.set_extensions(t_hot, k_ratio, t_cold, noise_amp, noise_seed)
+        if (t_hot != 0.0 or k_ratio != 1.0 or t_cold is not None or noise_amp != 0.0
+                or t_crit != 0.0 or t_ref != 0.0):
+            self.set_extensions(t_hot, k_ratio, t_cold, noise_amp, noise_seed, t_crit, t_ref)
         if initialize:
             self.initialization()
 
     def set_extensions(self, t_hot: float = 0.0, k_ratio: float = 1.0,
                        t_cold: float | None = None, noise_amp: float = 0.0,
-                       noise_seed: int = 0) -> None:
+                       noise_seed: int = 0, t_crit: float = 0.0, t_ref: float = 0.0) -> None:
         e = BgkExtC(t_hot, k_ratio, 0 if t_cold is None else 1, 0,
-                    0.0 if t_cold is None else t_cold, noise_amp, noise_seed)
+                    0.0 if t_cold is None else t_cold, noise_amp, noise_seed, t_crit, t_ref)
         _check(self._lib.bgk_set_extensions(self._h, ctypes.byref(e)))
 
     # ---- adjoint (FP64 moment handles): probe-temperature record and its

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define BGK_ABI_VERSION 2
+#define BGK_ABI_VERSION 3
 
 /* return codes */
 enum {
@@ -135,7 +135,8 @@ typedef struct bgk_info {
 } bgk_info;
 
 /* Physics extensions (not in the reference; the defaults t_hot = 0 ->
- * T0 and k_ratio = 1 give STEP_7's step).  Honoured by moment handles
+ * T0, k_ratio = 1, east_cold = 0, noise_amp = 0, t_crit = 0 -> 273 and
+ * t_ref = 0 give STEP_7's step).  Honoured by moment handles
  * (BGK_FP32_MOM, BGK_FP64_MOM); other handles accept only the defaults. */
 typedef struct bgk_ext {
   double t_hot;       /* hot (west) wall and initial liquid temperature in
@@ -154,6 +155,12 @@ typedef struct bgk_ext {
                          noise_seed + (q+1) 0x9E3779B97F4A7C15, q = i +
                          (Nx+2) j; 0: none                                  */
   uint64_t noise_seed;
+  double t_crit;      /* critical temperature of the phase field in place of
+                         STEP_7/phase_field.f90:12's 273 (0: 273)           */
+  double t_ref;       /* buoyancy reference temperature: forces.f90:20 with
+                         T - t_ref in place of T (the Boussinesq form: no
+                         hydrostatic part at T = t_ref); 0: STEP_7's
+                         absolute T                                          */
 } bgk_ext;
 
 typedef struct bgk_handle bgk_handle;

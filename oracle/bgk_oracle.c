@@ -38,6 +38,7 @@ struct oracle {
   int east_cold;
   double t_cold, noise_amp;
   uint64_t noise_seed;
+  double t_crit, t_ref; /* phase-field T_crit (273), buoyancy reference (0) */
   double* fld[BGK_NFIELDS];
 };
 
@@ -81,6 +82,8 @@ oracle_t* oracle_create(const bgk_params* p) {
   s->interf = p->nx / 40;
   s->t_hot = p->t0;
   s->k_ratio = 1.0;
+  s->t_crit = 273.0;
+  s->t_ref = 0.0;
   s->tin = s->t_hot;
   s->u0 = p->re * s->kvisc / p->length;
   s->v0 = 0.0;
@@ -111,6 +114,15 @@ void oracle_set_ext2(oracle_t* s, int east_cold, double t_cold, double noise_amp
   s->t_cold = t_cold;
   s->noise_amp = noise_amp;
   s->noise_seed = noise_seed;
+}
+
+/* Extensions: the phase field's critical temperature t_crit in place of
+ * phase_field.f90:12's 273, and a reference temperature t_ref of the
+ * buoyancy, forces.f90:20 with T - t_ref in place of T (Boussinesq form;
+ * 0 keeps STEP_7's absolute T).  Call before the first step. */
+void oracle_set_ext3(oracle_t* s, double t_crit, double t_ref) {
+  s->t_crit = t_crit;
+  s->t_ref = t_ref;
 }
 
 static double splitmix_uniform(uint64_t seed, uint64_t q) {
@@ -459,7 +471,7 @@ void oracle_moments_pre(oracle_t* s) { /* momentsPre.f90:8-21 */
 void oracle_phase_field(oracle_t* s) { /* phase_field.f90:15-52 */
   const int nx = s->nx, ny = s->ny, ld = s->ld;
   double *phi = s->fld[BGK_PHI], *pp = s->fld[BGK_PHI_PREC], *T = s->fld[BGK_T];
-  const double t_act = 0.0, t_crit = 273.0, f_plus = 0.5, f_minus = 0.5;
+  const double t_act = 0.0, t_crit = s->t_crit, f_plus = 0.5, f_minus = 0.5;
   const double tw = s->p.t_width;
   memcpy(pp, phi, s->n * sizeof(double)); /* :19 phi_prec(:,:) = phi(:,:) */
 #pragma omp parallel for schedule(static)
@@ -506,7 +518,7 @@ void oracle_forces(oracle_t* s) { /* forces.f90:12-26 */
     for (int i = 1; i <= nx; ++i) {
       size_t q = IX(i, j);
       double t = T[q] - (delta_t / st * (phi[q] - pp[q]) / 2.0);
-      double fy = rho[q] * grav * beta * t * (1.0 + phi[q] / 2.0);
+      double fy = rho[q] * grav * beta * (t - s->t_ref) * (1.0 + phi[q] / 2.0);
       double fx = -(1.0 * u[q] * (1.0 - phi[q]) * 0.5);
       fy = fy - 1.0 * v[q] * (1.0 - phi[q]) * 0.5;
       T[q] = t;

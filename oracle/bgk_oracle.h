@@ -46,6 +46,9 @@ void oracle_set_ext(oracle_t* s, double t_hot, double k_ratio);
  * bgk_ext in include/bgk_pcm.h). */
 void oracle_set_ext2(oracle_t* s, int east_cold, double t_cold, double noise_amp,
                      uint64_t noise_seed);
+/* Phase-field critical temperature (273 in STEP_7) and buoyancy reference
+ * temperature (0: STEP_7's absolute T). */
+void oracle_set_ext3(oracle_t* s, double t_crit, double t_ref);
 
 /* STEP_7/initialization.f90:5-152 */
 void oracle_initialization(oracle_t* s);

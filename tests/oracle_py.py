@@ -61,6 +61,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_set_ext.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.oracle_set_ext2.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_uint64]
+        L.oracle_set_ext3.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.oracle_tanh.restype = ctypes.c_double
         L.oracle_tanh.argtypes = [ctypes.c_double]
         _lib = L
@@ -72,7 +73,7 @@ class Oracle:
 
     def __init__(self, p: Params, initialize: bool = True, t_hot: float | None = None,
                  k_ratio: float = 1.0, t_cold: float | None = None, noise_amp: float = 0.0,
-                 noise_seed: int = 0):
+                 noise_seed: int = 0, t_crit: float = 273.0, t_ref: float = 0.0):
         self.p = p
         self.nx, self.ny = p.nx, p.ny
         self._c = p.to_c()
@@ -84,6 +85,8 @@ class Oracle:
         if t_cold is not None or noise_amp != 0.0:
             lib().oracle_set_ext2(self.h, 0 if t_cold is None else 1,
                                   0.0 if t_cold is None else t_cold, noise_amp, noise_seed)
+        if t_crit != 273.0 or t_ref != 0.0:
+            lib().oracle_set_ext3(self.h, t_crit, t_ref)
         if initialize:
             lib().oracle_initialization(self.h)
 

@@ -205,3 +205,38 @@ def test_adjoint_with_cold_wall_and_noise_matches_central_differences():
         fd = (loss(rec_at(tp)) - loss(rec_at(tm))) / (2 * h)
         print(f"component {c}: adjoint {g[c]:.10e} central difference {fd:.10e}")
         assert abs(fd - g[c]) <= 1e-4 * abs(g[c]), (c, fd, g[c])
+
+
+def test_adjoint_with_critical_and_reference_temperatures_matches_central_differences():
+    """T_crit and the Boussinesq buoyancy reference change the phase-field
+    rates and the buoyancy term of the reversed update."""
+    p = Params(nx=40, ny=32, length=40.0, grav=2e-3, rounding=ROUND_CLEAN, stefan_number=0.1)
+    nsteps, seg = 200, 30
+    pi, pj = seeded_probes(p.nx, p.ny, 8, seed=6)
+    kw = dict(t_crit=274.0, t_ref=278.0)
+
+    def rec_at(theta, keep=False):
+        st, kr, th = theta
+        lat = Lattice(p.replace(precision=FP64_MOM, stefan_number=st), initialize=False,
+                      t_hot=th, k_ratio=kr, **kw)
+        lat.initialization()
+        lat.adj_setup(nsteps, seg, pi, pj)
+        rec = lat.adj_forward()
+        if keep:
+            return lat, rec
+        lat.close()
+        return rec
+
+    target = rec_at((0.12, 1.2, 283.2))
+    theta0 = np.array([0.1, 1.5, 283.5])
+    lat, rec = rec_at(theta0, keep=True)
+    g = lat.adj_backward(2.0 * (rec - target) / rec.size)
+    lat.close()
+    loss = lambda r: float(((r - target) ** 2).mean())
+    for c, h in enumerate((1e-6, 1e-6, 1e-4)):
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[c] += h
+        tm[c] -= h
+        fd = (loss(rec_at(tp)) - loss(rec_at(tm))) / (2 * h)
+        print(f"component {c}: adjoint {g[c]:.10e} central difference {fd:.10e}")
+        assert abs(fd - g[c]) <= 1e-4 * abs(g[c]), (c, fd, g[c])

@@ -166,15 +166,39 @@ def test_extensions_match_oracle(precision, tol, t_hot, k_ratio):
     assert max(err.values()) <= tol, err
 
 
+@pytest.mark.parametrize("precision,tol", [(FP64_MOM, 1e-10), (FP32_MOM, 1e-6)])
+@pytest.mark.parametrize("t_crit,t_ref", [(275.0, 0.0), (273.0, 278.0), (271.5, 273.0)])
+def test_critical_and_reference_temperatures_match_oracle(precision, tol, t_crit, t_ref):
+    """The phase field's T_crit and the buoyancy's reference temperature
+    (Boussinesq form) against the oracle, 300 steps on 48 x 40."""
+    p = Params(nx=48, ny=40, length=48.0, grav=1e-3, rounding=ROUND_CLEAN)
+    o = Oracle(p, t_crit=t_crit, t_ref=t_ref)
+    o.step(300)
+    with Lattice(p.replace(precision=precision), t_crit=t_crit, t_ref=t_ref) as lat:
+        lat.step(300)
+        err = rel_errors(lat, o, interior=False, speed_scale=True)
+    print(f"t_crit={t_crit} t_ref={t_ref}: {err}")
+    assert max(err.values()) <= tol, err
+
+
 def test_extensions_need_a_moment_handle():
     with Lattice(Params(nx=8, ny=8)) as lat:
         lat.set_extensions(0.0, 1.0)  # the defaults are accepted everywhere
         with pytest.raises(BgkError) as e:
             lat.set_extensions(284.0, 1.0)
         assert e.value.code == -4
+    with Lattice(Params(nx=8, ny=8)) as lat:
+        lat.set_extensions(t_crit=273.0)  # T_crit's STEP_7 value is a default
+        for kw in (dict(t_crit=275.0), dict(t_ref=278.0)):
+            with pytest.raises(BgkError) as e:
+                lat.set_extensions(**kw)
+            assert e.value.code == -4, kw
     with Lattice(Params(nx=8, ny=8, precision=FP64_MOM)) as lat:
         with pytest.raises(BgkError) as e:
             lat.set_extensions(284.0, 0.0)
+        assert e.value.code == -1
+        with pytest.raises(BgkError) as e:
+            lat.set_extensions(t_ref=-1.0)
         assert e.value.code == -1
 
 

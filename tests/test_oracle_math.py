@@ -17,7 +17,9 @@ reference build: each would fail for a plausible slip in the restatement
  - moments are the averages of the pre- and post-collision moments;
  - field_check / mass_check / diag sums equal numpy's;
  - the 32-bit Ny**3 of diag.f90 wraps to 0 at Ny = 2048;
- - melting is monotone in a conduction-only run (physics sanity).
+ - melting is monotone in a conduction-only run (physics sanity);
+ - extensions: the phase field depends on T - T_crit only; the buoyancy
+   vanishes at T = t_ref and scales as T - t_ref.
 """
 import numpy as np
 import pytest
@@ -273,3 +275,49 @@ def test_cold_east_wall_sets_the_east_ghost_equilibrium():
         cu = CX[k] * u + CY[k] * v
         eq = W[k] * 270.0 * (1 + 3 * cu + 4.5 * cu * cu - uu)
         np.testing.assert_allclose(o.field(f"g{k}")[1:-1, -1], eq, rtol=1e-15)
+
+
+def test_critical_temperature_shifts_the_phase_field():
+    """The phase field depends on T - T_crit only: T_crit = 283 on T equals the
+    default 273 on T - 10 bit for bit (temperatures on a 1/64 grid, so the
+    shift is exact)."""
+    rng = np.random.default_rng(21)
+    shape = (22, 22)
+    T = 263.0 + rng.integers(0, 64 * 40, size=shape) / 64.0
+    phi0 = rng.uniform(-1.0, 1.0, size=shape)
+    out = []
+    for t_crit, shift in ((283.0, 0.0), (273.0, 10.0)):
+        o = Oracle(Params(nx=20, ny=20), initialize=False, t_crit=t_crit)
+        o.field("phi")[...] = phi0
+        o.field("T")[...] = T - shift
+        o.stage("phase_field")
+        out.append(o.field("phi").copy())
+    assert np.array_equal(out[0], out[1])
+    o = Oracle(Params(nx=20, ny=20), initialize=False)
+    o.field("phi")[...] = phi0
+    o.field("T")[...] = T
+    o.stage("phase_field")
+    assert not np.array_equal(o.field("phi"), out[0])  # the shift matters
+
+
+def test_buoyancy_reference_temperature():
+    """forces with t_ref: a liquid node at rest at T = t_ref feels no
+    buoyancy, and the velocity increment scales as T - t_ref."""
+    p = Params(nx=10, ny=8, tau_f=1.0, grav=1e-3)
+
+    def dv(T, t_ref):
+        o = Oracle(p, initialize=False, t_ref=t_ref)
+        o.field("rho")[...] = 1.0
+        o.field("T")[...] = T
+        o.field("u")[...] = 0.0
+        o.field("v")[...] = 0.0
+        o.field("phi")[...] = 1.0
+        o.field("phi_prec")[...] = 1.0
+        o.stage("forces")
+        return o.field("v")[1:-1, 1:-1].copy()
+
+    assert np.all(dv(278.0, 278.0) == 0.0)
+    a, b = dv(283.0, 278.0), dv(283.0, 273.0)
+    assert np.all(a > 0.0)
+    np.testing.assert_allclose(a / b, (283.0 - 278.0) / (283.0 - 273.0), rtol=1e-14)
+    np.testing.assert_allclose(dv(283.0, 0.0) / b, 283.0 / 10.0, rtol=1e-14)
