@@ -45,6 +45,7 @@ Printed JSON line (one):
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
                        [--error-steps E] [--cpu-steps S] [--no-cpu-baseline]
                        [--no-fp64] [--no-adjoint] [--no-adjoint-large]
+                       [--adjoint-large-steps N] [--adjoint-large-only]
                        [--calibrate-copy MiB]
 """
 from __future__ import annotations
@@ -83,6 +84,10 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-adjoint-large", action="store_true")
+    ap.add_argument("--adjoint-large-steps", type=int, default=5000)
+    ap.add_argument("--adjoint-large-only", action="store_true",
+                    help="only the large adjoint workload, without the 1024^2 check "
+                         "(for profiling it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="after the timed region, run a device-to-device copy of this size "
                          "(reference kernel for HBM counter scaling in profiles)")
@@ -277,7 +282,7 @@ def progress(msg: str) -> None:
 
 
 def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: int = 17,
-                  n_probes: int = 16, check_n: int = 1024):
+                  n_probes: int = 16, check_n: int | None = 1024):
     """The large adjoint workload: FP32 moment storage at n^2 (cavity of the
     bench, Ra 1e7, St 0.1) over nsteps steps with two levels of checkpoints
     (about nsteps^(1/3) steps apart in the inner level); the loss is the
@@ -313,7 +318,8 @@ def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: 
         return g, tf, tb, adj_bytes, nstates, handle_bytes
 
     out = {}
-    for size, prec_list in ((n, (FP32_MOM,)), (check_n, (FP32_MOM, FP64_MOM))):
+    runs = [(n, (FP32_MOM,))] + ([(check_n, (FP32_MOM, FP64_MOM))] if check_n else [])
+    for size, prec_list in runs:
         base = cavity(size).replace(rounding=ROUND_CLEAN, stefan_number=0.1)
         dT = base.t0 - base.t_in_sol
         theta0 = (0.1, 1.0, base.t0)
@@ -336,8 +342,7 @@ def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: 
         lat.step(nsteps)
         lat.synchronize()
         t_plain = (time.perf_counter() - t0) / nsteps
-    g32, g64 = out[(check_n, FP32_MOM)][0], out[(check_n, FP64_MOM)][0]
-    return {"workload": f"{n}x{n} cavity, FP32 moments, Ra 1e7, St 0.1, {nsteps} steps, "
+    res = {"workload": f"{n}x{n} cavity, FP32 moments, Ra 1e7, St 0.1, {nsteps} steps, "
                         f"checkpoints every {seg_outer} and {seg} steps, {n_probes} probes",
             "grad": [float(x) for x in g],
             "states_stored": nstates,
@@ -346,11 +351,14 @@ def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: 
             "forward_ms_per_step": round(t_plain * 1e3, 4),
             "recorded_forward_ms_per_step": round(tf / nsteps * 1e3, 4),
             "backward_ms_per_step": round(tb / nsteps * 1e3, 4),
-            "backward_over_forward": round(tb / nsteps / t_plain, 3),
-            "check": {"size": f"{check_n}x{check_n}", "steps": nsteps,
-                      "grad_fp32_moments": [float(x) for x in g32],
-                      "grad_fp64_moments": [float(x) for x in g64],
-                      "rel_difference": [float(abs(a - b) / abs(b)) for a, b in zip(g32, g64)]}}
+            "backward_over_forward": round(tb / nsteps / t_plain, 3)}
+    if check_n:
+        g32, g64 = out[(check_n, FP32_MOM)][0], out[(check_n, FP64_MOM)][0]
+        res["check"] = {"size": f"{check_n}x{check_n}", "steps": nsteps,
+                        "grad_fp32_moments": [float(x) for x in g32],
+                        "grad_fp64_moments": [float(x) for x in g64],
+                        "rel_difference": [float(abs(a - b) / abs(b)) for a, b in zip(g32, g64)]}
+    return res
 
 
 def measure(p, stream, steps, warmup, kind, kernel):
@@ -378,6 +386,10 @@ def main():
     from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, ROUND_CLEAN
 
     torch.cuda.set_device(0)
+    if args.adjoint_large_only:
+        print(json.dumps({"adjoint_large": adjoint_large(nsteps=args.adjoint_large_steps,
+                                                         check_n=None)}), flush=True)
+        return
     stream = torch.cuda.Stream()
     n = args.size
     p = cavity(n)
@@ -439,7 +451,7 @@ def main():
         progress("adjoint 512^2")
         res["adjoint"] = adjoint_run()
     if not args.no_adjoint_large:
-        res["adjoint_large"] = adjoint_large()
+        res["adjoint_large"] = adjoint_large(nsteps=args.adjoint_large_steps)
     if args.calibrate_copy > 0:
         # copies of known size for counter scaling: the runtime's device copy,
         # then 8-byte plain and non-temporal copies (libbgk_calib.so), the
