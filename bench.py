@@ -379,7 +379,7 @@ def measure(p, stream, steps, warmup, kind, kernel):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the time step runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is design-only)", file=sys.stderr)
         sys.exit(2)
     import torch

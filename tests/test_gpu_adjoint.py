@@ -8,8 +8,8 @@ gradient with respect to theta = (stefan_number, k_ratio, t_hot).
 Tolerances: probe record vs the CPU reference 1e-11 relative (FP64 moment
 form vs population form: only the order of roundings differs); gradient
 vs autograd 1e-7 relative per component; gradient vs central differences
-(steps 1e-6 St, 1e-6 k_ratio, 1e-4 t_hot) 1e-4 relative, the north star's
-bar.
+(steps 1e-6 St, 1e-6 k_ratio, 1e-4 t_hot) 1e-4 relative, the bar the
+adjoint's gradient is held to.
 """
 import numpy as np
 import pytest
