@@ -11,7 +11,8 @@
 // Adjoint state per node (step n): the adjoints of the ten pulled moments
 // the node update depends on (PullSums of bgk_mom.h: rho, j, the second
 // moments of f; T and the second moments of g) and of the two tap sums and
-// the centre tap of the phase field, 13 doubles.  The pull fp_k(x) =
+// the centre tap of the phase field: 13 values, 10 with omega_f = 1, stored
+// in the state's precision and computed in FP64.  The pull fp_k(x) =
 // f_k(x - c_k) is reversed as a gather, so no atomics: node y collects,
 // for every direction k, the adjoint of the population k that node y + c_k
 // pulled from it, and turns it into the adjoint of its own stored moments
@@ -47,10 +48,19 @@ struct StatePtrs {
   double* te;
 };
 
-// adjoint planes of one step (structure of arrays, node q at [q])
+// adjoint planes of one step (structure of arrays, node q at [q]), stored
+// in the state's precision R (the arithmetic is FP64 either way)
 struct AdjPlanes {
-  double* v[13];  // rho jx jy fxx fyy fxy dT gxx gyy gxy kp km pc
+  void* v[13];  // rho jx jy fxx fyy fxy dT gxx gyy gxy kp km pc
 };
+template <typename R>
+__device__ __forceinline__ double plane_at(const AdjPlanes& L, int p, int64_t q) {
+  return (double)static_cast<const R*>(L.v[p])[q];
+}
+template <typename R>
+__device__ __forceinline__ void plane_set(const AdjPlanes& L, int p, int64_t q, double x) {
+  static_cast<R*>(L.v[p])[q] = (R)x;
+}
 enum { L_RHO, L_JX, L_JY, L_FXX, L_FYY, L_FXY, L_DT, L_GXX, L_GYY, L_GXY, L_KP, L_KM, L_PC };
 
 struct AdjArgs {
@@ -344,7 +354,7 @@ __global__ __launch_bounds__(BX* BY) void k_adjoint(AdjArgs a) {
       const int64_t x = (int64_t)xj * P + xi;
       if (a.has_next)
 #pragma unroll
-        for (int sl = 0; sl < S::N; ++sl) W.lam[sl][ey][ex] = a.next.v[S::plane[sl]][x];
+        for (int sl = 0; sl < S::N; ++sl) W.lam[sl][ey][ex] = plane_at<R>(a.next, S::plane[sl], x);
       W.a[ey][ex] = a.sprev.a[x];
       W.b[ey][ex] = b_at<R, EQF>(a.sprev.b, x);
       W.c[ey][ex] = c_at<R, EQF>(a.sprev.c, x);
@@ -437,7 +447,7 @@ __global__ __launch_bounds__(BX* BY) void k_adjoint(AdjArgs a) {
           }
         }
       }
-      lphi += a.next.v[L_PC][q];
+      lphi += plane_at<R>(a.next, L_PC, q);
     }
     if (a.core) {
       // ---- loss seed at a probe node (averaged T after step n)
@@ -455,22 +465,22 @@ __global__ __launch_bounds__(BX* BY) void k_adjoint(AdjArgs a) {
       PullSums lS;
       double lkp, lkm, lpc;
       core_backward(a.c, S, tap, o, lm, lphi, 0.5 * lTrep, &lS, &lkp, &lkm, &lpc, g);
-      double* const* v = a.cur.v;
-      v[L_RHO][q] = lS.rho;
-      v[L_JX][q] = lS.jx;
-      v[L_JY][q] = lS.jy;
+      const AdjPlanes& v = a.cur;
+      plane_set<R>(v, L_RHO, q, lS.rho);
+      plane_set<R>(v, L_JX, q, lS.jx);
+      plane_set<R>(v, L_JY, q, lS.jy);
       if constexpr (!EQF) {
-        v[L_FXX][q] = lS.fxx;
-        v[L_FYY][q] = lS.fyy;
-        v[L_FXY][q] = lS.fxy;
+        plane_set<R>(v, L_FXX, q, lS.fxx);
+        plane_set<R>(v, L_FYY, q, lS.fyy);
+        plane_set<R>(v, L_FXY, q, lS.fxy);
       }
-      v[L_DT][q] = lS.dT;
-      v[L_GXX][q] = lS.gxx;
-      v[L_GYY][q] = lS.gyy;
-      v[L_GXY][q] = lS.gxy;
-      v[L_KP][q] = lkp;
-      v[L_KM][q] = lkm;
-      v[L_PC][q] = lpc;
+      plane_set<R>(v, L_DT, q, lS.dT);
+      plane_set<R>(v, L_GXX, q, lS.gxx);
+      plane_set<R>(v, L_GYY, q, lS.gyy);
+      plane_set<R>(v, L_GXY, q, lS.gxy);
+      plane_set<R>(v, L_KP, q, lkp);
+      plane_set<R>(v, L_KM, q, lkm);
+      plane_set<R>(v, L_PC, q, lpc);
     } else if (i <= a.interf) {
       // initial state: T = t_hot in the liquid strip (initialization.f90:15-26)
       g[2] += lm.th;
@@ -559,6 +569,9 @@ namespace {
 size_t b_bytes(const bgk_handle* h) {
   return h->prm.precision == BGK_FP64_MOM ? sizeof(double4) : sizeof(float4);
 }
+size_t lam_bytes(const bgk_handle* h) {
+  return h->prm.precision == BGK_FP64_MOM ? sizeof(double) : sizeof(float);
+}
 size_t c_bytes(const bgk_handle* h) {
   const bool f64 = h->prm.precision == BGK_FP64_MOM;
   if (h->mom_eqf) return f64 ? sizeof(double) : sizeof(float);
@@ -609,7 +622,7 @@ size_t carve_ctx(bgk_handle* h, AdjCtx* a, char* base) {
   for (int l = 0; l < 2; ++l)
     for (int v = 0; v < 13; ++v) {
       const bool zero = h->mom_eqf && (v == L_FXX || v == L_FYY || v == L_FXY);
-      a->lam[l].v[v] = zero ? nullptr : b.plane<double>(n);
+      a->lam[l].v[v] = zero ? nullptr : b.plane<char>(n, lam_bytes(h));
     }
   a->probe_idx = b.plane<int>(n);
   a->probe_ij = b.take<int>(2 * (size_t)std::max(a->np, 1));
