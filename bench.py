@@ -30,6 +30,9 @@ Printed JSON line (one):
                  at St 0.12, T_h 1.05, k_ratio 1.5), central differences
                  beside it, and the cost of the backward pass (recompute +
                  adjoint) per step over the forward step
+  slab           the moment step on one GPU's share of an 8-GPU slab
+                 decomposition of 8192^2: 8192 x 1026 nodes (1024 rows and
+                 the two rows a neighbour would send), same measurements
   adjoint_large  the large adjoint workload (8192^2, FP32 moment storage,
                  Ra 1e7, St 0.1, 5000 steps, two levels of checkpoints every
                  289 and 17 steps, 16 seeded probes, the same kind of target
@@ -173,15 +176,28 @@ def fp32_error(p64, storages: dict, steps: int):
     return out
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(p, steps: int):
     from bgk_pcm_amd.params import write_input_dat
     exe = ROOT / "oracle" / "_ref" / "openmp" / "ref_bench"
-    sample = f"{p.nx}x{p.ny} cavity, {steps} steps (same input as the GPU run)"
+    sample = f"{p.nx}x{p.ny} cavity, {steps} steps (same input as the GPU run), {cpu_model()}"
     if exe.exists():
+        env = dict(os.environ)
+        env.setdefault("OMP_PROC_BIND", "spread")
+        env.setdefault("OMP_PLACES", "cores")
         with tempfile.TemporaryDirectory(prefix="bgkcpu_") as wd:
             write_input_dat(p, os.path.join(wd, "input.dat"))
             r = subprocess.run([str(exe), str(steps)], cwd=wd, capture_output=True, text=True,
-                               timeout=900)
+                               timeout=900, env=env)
             if r.returncode != 0:
                 raise RuntimeError(f"ref_bench failed: {r.stderr[-500:]}")
             line = [l for l in r.stdout.splitlines() if l.startswith("nx=")][-1]
@@ -447,6 +463,10 @@ def main():
     if args.error_steps > 0:
         progress(f"FP32 storage errors after {args.error_steps} steps")
         res["fp32_error"] = fp32_error(p, {"mom": pm, "fp32_population": pp}, args.error_steps)
+    # one GPU's share of the 8192^2 lattice over 8 GPUs (design-only sharding)
+    ps = cavity(8192).replace(ny=1026, precision=FP32_MOM, rounding=ROUND_CLEAN)
+    res["slab"] = dict(measure(ps, stream, args.steps, args.warmup, mom_kind, "k_fused_mom"),
+                       workload="8192 x 1026: one GPU's rows of 8192^2 on 8 GPUs + 2 halo rows")
     if not args.no_adjoint:
         progress("adjoint 512^2")
         res["adjoint"] = adjoint_run()
