@@ -39,13 +39,12 @@ namespace {
 
 using namespace mom;
 
-// one stored state: moment planes, phi (in the moments' precision),
-// east-wall column
+// one stored state: moment planes, phi, east-wall column
 struct StatePtrs {
   double2* a;
   void* b;
   void* c;
-  void* phi;
+  double* phi;
   double* te;
 };
 
@@ -359,7 +358,7 @@ __global__ __launch_bounds__(BX* BY) void k_adjoint(AdjArgs a) {
       W.a[ey][ex] = a.sprev.a[x];
       W.b[ey][ex] = b_at<R, EQF>(a.sprev.b, x);
       W.c[ey][ex] = c_at<R, EQF>(a.sprev.c, x);
-      W.phi[ey][ex] = phi_load(static_cast<const R*>(a.sprev.phi)[x]);
+      W.phi[ey][ex] = a.sprev.phi[x];
     }
   }
   __syncthreads();
@@ -430,7 +429,7 @@ __global__ __launch_bounds__(BX* BY) void k_adjoint(AdjArgs a) {
       // ---- phi of step n as a tap of step n+1 (C, E, N, W, S, NE, NW, SW, SE)
       constexpr int DX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
       constexpr int DY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-      const double ph = phi_load(static_cast<const R*>(a.sn.phi)[q]);
+      const double ph = a.sn.phi[q];
       const double aa = 0.5 * (1.0 + ph), bb = 0.5 * (1.0 - ph);
 #pragma unroll
       for (int t = 0; t < 9; ++t) {
@@ -566,15 +565,13 @@ struct AdjCtx {
 
 namespace {
 
-// bytes per node of the B and C moment planes of the handle's storage, of
-// its phi plane and of an adjoint plane
+// bytes per node of the B and C moment planes of the handle's storage
 size_t b_bytes(const bgk_handle* h) {
   return h->prm.precision == BGK_FP64_MOM ? sizeof(double4) : sizeof(float4);
 }
 size_t lam_bytes(const bgk_handle* h) {
   return h->prm.precision == BGK_FP64_MOM ? sizeof(double) : sizeof(float);
 }
-size_t phi_bytes(const bgk_handle* h) { return lam_bytes(h); }
 size_t c_bytes(const bgk_handle* h) {
   const bool f64 = h->prm.precision == BGK_FP64_MOM;
   if (h->mom_eqf) return f64 ? sizeof(double) : sizeof(float);
@@ -609,7 +606,7 @@ void carve_state(Bump& b, const bgk_handle* h, StatePtrs* s) {
   s->a = b.plane<double2>(n);
   s->b = b.plane<char>(n, b_bytes(h));
   s->c = b.plane<char>(n, c_bytes(h));
-  s->phi = b.plane<char>(n, phi_bytes(h));
+  s->phi = b.plane<double>(n);
   s->te = b.take<double>((size_t)h->ny + 2);
 }
 
@@ -638,7 +635,7 @@ size_t carve_ctx(bgk_handle* h, AdjCtx* a, char* base) {
 }
 
 hipError_t copy_state(bgk_handle* h, const StatePtrs& dst, const double2* a, const void* bb,
-                      const void* c, const void* phi, const double* te) {
+                      const void* c, const double* phi, const double* te) {
   const size_t n = (size_t)h->plane_elems;
   hipError_t e = hipMemcpyAsync(dst.a, a, n * sizeof(double2), hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess)
@@ -646,7 +643,7 @@ hipError_t copy_state(bgk_handle* h, const StatePtrs& dst, const double2* a, con
   if (e == hipSuccess)
     e = hipMemcpyAsync(dst.c, c, n * c_bytes(h), hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(dst.phi, phi, n * phi_bytes(h), hipMemcpyDeviceToDevice, h->stream);
+    e = hipMemcpyAsync(dst.phi, phi, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(dst.te, te, ((size_t)h->ny + 2) * sizeof(double), hipMemcpyDeviceToDevice,
                        h->stream);
@@ -769,8 +766,8 @@ hipError_t adj_forward(bgk_handle* h, double* probe_T_host) {
   int te = 0;  // teast[te] holds the current column
   for (int n = 0; n < x->nsteps; ++n) {
     const int a = h->cur, b = 1 - h->cur;
-    StatePtrs in{h->ma[a], h->mb[a], h->mc[a], h->mp[a], h->teast[te]};
-    StatePtrs out{h->ma[b], h->mb[b], h->mc[b], h->mp[b], h->teast[1 - te]};
+    StatePtrs in{h->ma[a], h->mb[a], h->mc[a], h->phi[h->pc], h->teast[te]};
+    StatePtrs out{h->ma[b], h->mb[b], h->mc[b], h->phi[1 - h->pc], h->teast[1 - te]};
     const int lvl1 = x->seg1 > 0 ? x->seg1 : x->seg;
     if (n % lvl1 == 0) {
       e = copy_state(h, x->ckpt[n / lvl1], in.a, in.b, in.c, in.phi, in.te);
@@ -780,6 +777,7 @@ hipError_t adj_forward(bgk_handle* h, double* probe_T_host) {
                  nullptr);
     if (e != hipSuccess) return e;
     h->cur = b;
+    h->pc = 1 - h->pc;
     te = 1 - te;
   }
   // the handle's teast convention: bgk_step recomputes it from T at entry

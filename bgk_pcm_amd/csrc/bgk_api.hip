@@ -95,7 +95,6 @@ size_t carve(bgk_handle* h, char* base) {
     h->ma[b] = nullptr;
     h->mb[b] = nullptr;
     h->mc[b] = nullptr;
-    h->mp[b] = nullptr;
     for (int k = 0; k < 9; ++k) {
       h->f[b][k] = h->g[b][k] = nullptr;
       h->f32[b][k] = h->g32[b][k] = nullptr;
@@ -106,12 +105,10 @@ size_t carve(bgk_handle* h, char* base) {
         h->mb[b] = cv.plane<double4>(n);
         h->mc[b] = h->mom_eqf ? static_cast<void*>(cv.plane<double>(n))
                               : static_cast<void*>(cv.plane<double4>(n));
-        h->mp[b] = cv.plane<double>(n);
       } else {
         h->mb[b] = cv.plane<float4>(n);
         h->mc[b] = h->mom_eqf ? static_cast<void*>(cv.plane<float>(n))
                               : static_cast<void*>(cv.plane<float4>(n));
-        h->mp[b] = cv.plane<float>(n);
       }
       continue;
     }

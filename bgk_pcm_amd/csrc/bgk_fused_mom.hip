@@ -2,13 +2,13 @@
 //
 // The node state is the ten moments of bgk_mom.h (rho, T - T_crit in FP64;
 // u, v and the scaled non-equilibrium second moments in R = float or
-// double) and phi in R, 52 bytes per node with FP32 R (40 with omega_f = 1)
-// instead of 152 for FP32 populations and FP64 phi, and the pull rebuilds
-// population k of the upstream node from its moments.  rho and T stay
-// FP64: the rounding of rho (not of u) is what drives the velocity error of
-// FP32 storage, since a neighbour's density error enters the momentum
-// balance through sum_k c_k w_k rho(x - c_k).  u, v and the
-// non-equilibrium moments carry their own relative precision.
+// double), 48 bytes per node with FP32 R (36 with omega_f = 1) instead of
+// 144 for FP32 populations, and the pull rebuilds population k of the
+// upstream node from its moments.  rho and T stay FP64: the rounding of rho
+// (not of u) is what drives the velocity error of FP32 storage, since a
+// neighbour's density error enters the momentum balance through
+// sum_k c_k w_k rho(x - c_k).  u, v and the non-equilibrium moments carry
+// their own relative precision.
 //
 // Tiling: a workgroup owns W columns and a strip of consecutive rows, and
 // walks the strip row by row.  Row j+2 is fetched from HBM while row j is
@@ -34,10 +34,8 @@ using namespace mom;
 
 constexpr int TX = 64, TY = 4;  // block of the initialization kernel
 
-// Macroscopic output planes; phi and phip (phi_prec) may be null.
 struct Macro {
   double *rho, *u, *v, *T;
-  double *phi, *phip;
 };
 
 // LDS ring of NS rows of W + 2 nodes (W columns and the two halo columns).
@@ -46,7 +44,7 @@ struct Ring {
   double2 a[NS][W + 2];
   typename Vec<R>::V4 b[NS][W + 2];
   typename CType<R, EQF>::T c[NS][W + 2];
-  double phi[NS][W + 2];  // decoded once per node (phi_load)
+  double phi[NS][W + 2];
 };
 
 // Row strips: workgroup b runs on XCD b % 8; all column segments of a strip
@@ -60,13 +58,13 @@ struct StripGrid {
 };
 
 template <typename R, bool EQF>
-__device__ __forceinline__ Node<R, EQF> load_node(const MomIn& in, const R* __restrict__ pa,
+__device__ __forceinline__ Node<R, EQF> load_node(const MomIn& in, const double* __restrict__ pa,
                                                   const Geom& gm, int ic, int jr) {
   Node<R, EQF> n;
   n.a = make_double2(0.0, 0.0);
   n.b = make4<R>(R(0), R(0), R(0), R(0));
   n.c = typename CType<R, EQF>::T{};
-  n.phi = R(0);
+  n.phi = 0.0;
   const int64_t q = (int64_t)jr * gm.pitch + ic;
   if (ic >= 1 && ic <= gm.nx) {
     n.a = stream_load(in.a + q);
@@ -82,17 +80,16 @@ __device__ __forceinline__ void put(Ring<W, NS, R, EQF>& L, int r, int s, const 
   L.a[r][s] = n.a;
   L.b[r][s] = n.b;
   L.c[r][s] = n.c;
-  L.phi[r][s] = phi_load(n.phi);
+  L.phi[r][s] = n.phi;
 }
 
 // Node update from its pulled moments and phi_prec taps (C, E, N, W, S, NE,
 // NW, SW, SE); stores the node's moments and phi, and T (averaged as
-// STEP_7/moments.f90:11-25) at a probe node.  Returns the averaged moments
-// and the stored phi.
+// STEP_7/moments.f90:11-25) at a probe node.  Returns the averaged moments.
 template <typename R, bool EQF>
 __device__ __forceinline__ NodeOut update_node(const Consts& c, const PullSums& S,
                                                const double tap[9], const MomOut& out,
-                                               R* __restrict__ pb, int64_t q,
+                                               double* __restrict__ pb, int64_t q,
                                                const int* __restrict__ probe_idx,
                                                double* __restrict__ probe_out) {
   const Core o = core_forward(c, S, tap);
@@ -110,10 +107,8 @@ __device__ __forceinline__ NodeOut update_node(const Consts& c, const PullSums& 
     nt_store4<R>(static_cast<V4*>(out.b) + q, us, vs, (R)(omf1 * o.qfxx), (R)(omf1 * o.qfyy));
     nt_store4<R>(static_cast<V4*>(out.c) + q, (R)(omf1 * o.qfxy), pgxx, pgyy, pgxy);
   }
-  const R ph = phi_store<R>(o.phi);
-  __builtin_nontemporal_store(ph, pb + q);
+  stream_store(&pb[q], o.phi);
   NodeOut r;
-  r.phi = phi_load(ph);
   r.rho = o.rho;
   r.Tav = kTcrit + (o.dT + o.dT2) * 0.5;
   r.uav = ((double)us + o.u2) * 0.5;
@@ -131,8 +126,8 @@ __device__ __forceinline__ NodeOut update_node(const Consts& c, const PullSums& 
 // boundary-adjacent nodes (mom_edge_nodes).
 template <int W, int NS, typename R, bool EQF>
 __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
-                                             const R* __restrict__ pa,
-                                             R* __restrict__ pb, const Macro& m,
+                                             const double* __restrict__ pa,
+                                             double* __restrict__ pb, const Macro& m,
                                              const Consts& c, const Geom& gm, const StripGrid& sg,
                                              int write_all, const int* __restrict__ probe_idx,
                                              double* __restrict__ probe_out) {
@@ -204,10 +199,6 @@ __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
         m.T[q] = o.Tav;
         m.u[q] = o.uav;
         m.v[q] = o.vav;
-        if (m.phi) {
-          m.phi[q] = o.phi;
-          m.phip[q] = tap[0];
-        }
       }
     }
     if constexpr (NS == 3) __syncthreads();  // row j+2 overwrites row j-1
@@ -244,8 +235,8 @@ __device__ __forceinline__ bool edge_node(int e, int nx, int ny, int* i, int* j)
 // then the halo bookkeeping of the macroscopic fields.
 template <typename R, bool EQF>
 __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& out,
-                                               const R* __restrict__ pa,
-                                               R* __restrict__ pb,
+                                               const double* __restrict__ pa,
+                                               double* __restrict__ pb,
                                                const double* __restrict__ te_in,
                                                double* __restrict__ te_out, const Macro& m,
                                                const Consts& c, const Geom& gm, int e,
@@ -285,11 +276,8 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
   const int64_t rn = (int64_t)(j < ny ? j + 1 : ny) * P;
   const int64_t rs = (int64_t)(j > 1 ? j - 1 : 1) * P;
   const int64_t r0 = (int64_t)j * P;
-  const double tap[9] = {phi_load(pa[q]),          phi_load(pa[r0 + i + 1]),
-                         phi_load(pa[rn + i]),     phi_load(pa[r0 + i - 1]),
-                         phi_load(pa[rs + i]),     phi_load(pa[rn + i + 1]),
-                         phi_load(pa[rn + i - 1]), phi_load(pa[rs + i - 1]),
-                         phi_load(pa[rs + i + 1])};
+  const double tap[9] = {pa[q],          pa[r0 + i + 1], pa[rn + i],     pa[r0 + i - 1], pa[rs + i],
+                         pa[rn + i + 1], pa[rn + i - 1], pa[rs + i - 1], pa[rs + i + 1]};
   PullSums S;
   pull_zero(S);
 #pragma unroll
@@ -297,24 +285,14 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
   const NodeOut o = update_node<R, EQF>(c, S, tap, out, pb, q, probe_idx, probe_out);
 
   // ---- halo bookkeeping: BC_f N/S rows (rho copied, u = v = 0), BC_g
-  // T and phi ghosts, the east column's averaged T for the next step.  A
-  // phi ghost x takes phi_prec of node y (BC_g.f90:56-64 copies the rows
-  // before phase_field.f90:19 copies phi to phi_prec, so both hold it).
-  auto phi_ghost = [&](int64_t x, int64_t y) {
-    const R v = pa[y];
-    pb[x] = v;
-    if (m.phi) {
-      m.phi[x] = phi_load(v);
-      m.phip[x] = phi_load(v);
-    }
-  };
+  // T and phi ghosts, the east column's averaged T for the next step
   const bool w_edge = (i == 1), e_edge = (i == nx), s_edge = (j == 1), n_edge = (j == ny);
   if (s_edge) {
     m.rho[i] = m.rho[q];
     m.T[i] = m.T[q];
     m.u[i] = 0.0;
     m.v[i] = 0.0;
-    phi_ghost(i, q);
+    pb[i] = pa[q];
   }
   if (n_edge) {
     const int64_t h = (int64_t)(ny + 1) * P + i;
@@ -322,49 +300,45 @@ __device__ __forceinline__ void mom_edge_nodes(const MomIn& in, const MomOut& ou
     m.T[h] = m.T[q];
     m.u[h] = 0.0;
     m.v[h] = 0.0;
-    phi_ghost(h, q);
+    pb[h] = pa[q];
   }
   if (w_edge) {
     m.T[r0] = c.tin;
-    phi_ghost(r0, r0);
+    pb[r0] = pa[r0];
     if (s_edge) {
       m.T[0] = c.tin;
-      phi_ghost(0, P);
+      pb[0] = pa[P];
     }
     if (n_edge) {
       const int64_t h = (int64_t)(ny + 1) * P;
       m.T[h] = c.tin;
-      phi_ghost(h, (int64_t)ny * P);
+      pb[h] = pa[(int64_t)ny * P];
     }
   }
   if (e_edge) {
     const int64_t h = r0 + nx + 1;
     m.T[h] = east_T(c, te_in, j);
-    phi_ghost(h, h);
+    pb[h] = pa[h];
     te_out[j] = o.Tav;
     if (s_edge) {
       m.T[nx + 1] = east_T(c, te_in, 1);
-      phi_ghost(nx + 1, P + nx + 1);
+      pb[nx + 1] = pa[P + nx + 1];
     }
     if (n_edge) {
       const int64_t hn = (int64_t)(ny + 1) * P + nx + 1;
       m.T[hn] = east_T(c, te_in, ny);
-      phi_ghost(hn, (int64_t)ny * P + nx + 1);
+      pb[hn] = pa[(int64_t)ny * P + nx + 1];
     }
   }
   m.rho[q] = o.rho;
   m.T[q] = o.Tav;
   m.u[q] = o.uav;
   m.v[q] = o.vav;
-  if (m.phi) {
-    m.phi[q] = o.phi;
-    m.phip[q] = tap[0];
-  }
 }
 
 template <int W, int NS, typename R, bool EQF>
 __global__ __launch_bounds__(W) void k_fused_mom(
-    MomIn in, MomOut out, const R* __restrict__ pa, R* __restrict__ pb,
+    MomIn in, MomOut out, const double* __restrict__ pa, double* __restrict__ pb,
     const double* __restrict__ te_in, double* __restrict__ te_out, Macro m, Consts c, Geom gm,
     StripGrid sg, int write_all, const int* __restrict__ probe_idx,
     double* __restrict__ probe_out) {
@@ -379,8 +353,8 @@ __global__ __launch_bounds__(W) void k_fused_mom(
 // STEP_7/initialization.f90:15-152 for a moment handle: the initial
 // populations are equilibria, i.e. moments with zero non-equilibrium part.
 template <typename R, bool EQF>
-__global__ void k_initialization_mom(MomOut st, Macro m, R* pa, R* pb, Consts c, Geom gm,
-                                     int interf, int west_bc) {
+__global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* phip, Consts c,
+                                     Geom gm, int interf, int west_bc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > gm.nx + 1 || j > gm.ny + 1) return;
@@ -396,11 +370,8 @@ __global__ void k_initialization_mom(MomOut st, Macro m, R* pa, R* pb, Consts c,
   m.u[q] = s0.u;
   m.v[q] = s0.v;
   m.T[q] = s0.T;
-  const R ph = phi_store<R>(s0.phi);
-  pa[q] = ph;
-  pb[q] = ph;
-  m.phi[q] = phi_load(ph);
-  m.phip[q] = phi_load(ph);
+  phi[q] = s0.phi;
+  phip[q] = s0.phi;
 }
 
 // Row strips of about `target` rows, and at least one workgroup per slot
@@ -408,8 +379,7 @@ __global__ void k_initialization_mom(MomOut st, Macro m, R* pa, R* pb, Consts c,
 // lattices, down to 2 rows per strip.  The strip count is chosen, where one
 // exists within a factor two, as a multiple of 8 (the XCDs) whose workgroups
 // fill whole rounds of the device: measured at 4096^2, 64-row strips (512
-// workgroups) run at 60.7 GLUPS and 96-row strips (43 strips, 1.5 rounds) at
-// 45.
+// workgroups) run at 57 GLUPS and 96-row strips (43 strips, 1.5 rounds) at 43.
 StripGrid make_strip_grid(const bgk_handle* h, int w, int target, int slots) {
   StripGrid s;
   const int ny = h->ny;
@@ -427,15 +397,6 @@ StripGrid make_strip_grid(const bgk_handle* h, int w, int target, int slots) {
         break;
       }
     s.rows = std::max(1, (ny + pick - 1) / pick);
-    // one round of the device, if its strips are target..2 target rows:
-    // every CU streams one strip, with fewer boundary-row re-reads
-    // (measured at 4096^2: 128-row strips in one round 61.7 GLUPS, 64-row
-    // strips in two rounds 60.7)
-    if (slots > 0 && slots % s.nseg == 0 && (slots / s.nseg) % NXCD == 0) {
-      const int n1 = slots / s.nseg;
-      const int r1 = (ny + n1 - 1) / n1;
-      if (r1 >= target && r1 <= 2 * target) s.rows = r1;
-    }
   }
   s.nstrip = (ny + s.rows - 1) / s.rows;
   s.nmain = ((s.nstrip + NXCD - 1) / NXCD) * NXCD * s.nseg;
@@ -456,13 +417,11 @@ hipError_t launch_mom_t(bgk_handle* h, const MomStep& st) {
   const int nx = h->nx, ny = h->ny;
   const int nedge = nx * (ny > 1 ? 2 : 1) + (ny > 2 ? (ny - 2) * (nx > 1 ? 2 : 1) : 0);
   const int64_t nwg = (int64_t)sg.nmain + (nedge + W - 1) / W;
-  Macro m{h->rho, h->u, h->v, h->T, h->phi[h->pc], h->phi[1 - h->pc]};
-  if (st.macro[0])
-    m = Macro{st.macro[0], st.macro[1], st.macro[2], st.macro[3], nullptr, nullptr};
+  Macro m{h->rho, h->u, h->v, h->T};
+  if (st.macro[0]) m = Macro{st.macro[0], st.macro[1], st.macro[2], st.macro[3]};
   hipLaunchKernelGGL((k_fused_mom<W, NS, R, EQF>), dim3((unsigned)nwg), dim3(W), 0, h->stream,
-                     st.in, st.out, static_cast<const R*>(st.pa), static_cast<R*>(st.pb),
-                     st.te_in, st.te_out, m, h->c, h->geom, sg, st.write_all ? 1 : 0,
-                     st.probe_idx, st.probe_out);
+                     st.in, st.out, st.pa, st.pb, st.te_in, st.te_out, m, h->c, h->geom, sg,
+                     st.write_all ? 1 : 0, st.probe_idx, st.probe_out);
   return hipGetLastError();
 }
 
@@ -481,18 +440,18 @@ hipError_t launch_mom(bgk_handle* h, const MomStep& st) {
 
 template <typename R, bool EQF>
 void launch_init(bgk_handle* h, const MomOut& st, dim3 grd) {
-  Macro m{h->rho, h->u, h->v, h->T, h->phi[h->pc], h->phi[1 - h->pc]};
+  Macro m{h->rho, h->u, h->v, h->T};
   hipLaunchKernelGGL((k_initialization_mom<R, EQF>), grd, dim3(TX, TY), 0, h->stream, st, m,
-                     static_cast<R*>(h->mp[0]), static_cast<R*>(h->mp[1]), h->c, h->geom,
-                     h->geom.nx / 40, h->prm.west_bc);
+                     h->phi[h->pc], h->phi[1 - h->pc], h->c, h->geom, h->geom.nx / 40,
+                     h->prm.west_bc);
 }
 
 }  // namespace
 
 hipError_t launch_mom_step(bgk_handle* h, const MomStep& st) {
   // Tiling (workgroup width W, ring rows NS).  Measured at 4096^2 on MI355X
-  // with FP32 moments and FP64 phi: omega_f = 1 (36-byte nodes, 64-row
-  // strips) W = 512 / NS = 4 57.2 GLUPS, 512/3 53.1, 256/4 53.9 (32-row strips), 128/4 45.1,
+  // with FP32 moments: omega_f = 1 (36-byte nodes, 64-row strips) W = 512 /
+  // NS = 4 57.2 GLUPS, 512/3 53.1, 256/4 53.9 (32-row strips), 128/4 45.1,
   // 64/4 45.8; general layout (48 bytes, 32-row strips) 512/3 45.9, 512/4
   // 45.6, 256/3 42.0.  Wider segments stream longer contiguous runs of each
   // row.  FP64 moments take the same tiling with half the workgroups per CU.
@@ -516,8 +475,8 @@ hipError_t launch_fused_step_mom(bgk_handle* h, int step_parity, bool write_all)
   MomStep st;
   st.in = MomIn{h->ma[a], h->mb[a], h->mc[a]};
   st.out = MomOut{h->ma[b], h->mb[b], h->mc[b]};
-  st.pa = h->mp[a];
-  st.pb = h->mp[b];
+  st.pa = h->phi[h->pc];
+  st.pb = h->phi[1 - h->pc];
   st.te_in = h->teast[step_parity & 1];
   st.te_out = h->teast[(step_parity + 1) & 1];
   st.write_all = write_all;
@@ -525,7 +484,10 @@ hipError_t launch_fused_step_mom(bgk_handle* h, int step_parity, bool write_all)
   st.probe_out = nullptr;
   for (int k = 0; k < 4; ++k) st.macro[k] = nullptr;
   hipError_t e = launch_mom_step(h, st);
-  if (e == hipSuccess) h->cur = b;
+  if (e == hipSuccess) {
+    h->cur = b;
+    h->pc = 1 - h->pc;
+  }
   return e;
 }
 

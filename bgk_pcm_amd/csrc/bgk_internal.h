@@ -29,10 +29,7 @@ struct AdjCtx;  // adjoint context (bgk_adjoint.hip)
 // handles hold f32/g32 = f - w*rho0, g - w*273 (T_crit) the same way.
 // Moment handles hold per node ma = (rho, T - 273) FP64, mb = (u, v, Pf_xx,
 // Pf_yy) and mc = (Pf_xy, Pg_xx, Pg_yy, Pg_xy) FP32, double-buffered by cur
-// (with omega_f = 1: mb = (u, v, Pg_xx, Pg_yy), mc = Pg_xy), and phi in mp
-// in the same precision, also double-buffered by cur; their phi[pc] and
-// phi[1-pc] are the FP64 phi and phi_prec written with the macroscopic
-// fields (every node on the last step of a bgk_step call), and pc stays.
+// (with omega_f = 1: mb = (u, v, Pg_xx, Pg_yy), mc = Pg_xy).
 struct bgk_handle {
   bgk_params prm;
   bgk_ext ext;  // physics extensions (defaults: STEP_7)
@@ -57,7 +54,6 @@ struct bgk_handle {
   double2* ma[2];
   void* mb[2];    // float4 (FP32 moments) or double4 (FP64 moments) planes
   void* mc[2];    // the same, or float / double planes when mom_eqf
-  void* mp[2];    // phi: float (FP32 moments) or double (FP64 moments) planes
   bool mom_eqf;   // omega_f == 1: Pf vanishes, mb = (u, v, Pg_xx, Pg_yy), mc = Pg_xy
   int strip_rows;   // rows per workgroup strip of the moment kernel (0: default)
   int mom_variant;  // tiling variant of the moment kernel (0: default)

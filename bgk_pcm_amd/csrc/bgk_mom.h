@@ -10,8 +10,7 @@
 //   A = (rho, T - T_crit)                 double2
 //   B = (u, v, Pf_xx, Pf_yy)              R4 (float4 or double4)
 //   C = (Pf_xy, Pg_xx, Pg_yy, Pg_xy)      R4
-// and, when omega_f = 1 (Pf = 0, "EQF"): B = (u, v, Pg_xx, Pg_yy), C = Pg_xy;
-// phi is stored in R as well (phi_store / phi_load below).
+// and, when omega_f = 1 (Pf = 0, "EQF"): B = (u, v, Pg_xx, Pg_yy), C = Pg_xy.
 // The node update depends on its pulled populations only through ten
 // moments (PullSums), which is what makes the adjoint compact.
 #ifndef BGK_MOM_H
@@ -77,28 +76,6 @@ __device__ __forceinline__ void nt_store4(typename Vec<R>::V4* p, R x, R y, R z,
   __builtin_nontemporal_store(v, reinterpret_cast<typename Vec<R>::nt4*>(p));
 }
 
-// phi in R storage.  FP64 keeps phi itself.  FP32 keeps the signed distance
-// to the nearer phase value, e = -(1 - phi) for phi >= 0 (sign bit set,
-// -0 at phi = 1) and e = 1 + phi below, so that its rounding is relative to
-// 1 - |phi|.  A node approaches +-1 by increments of order (1 - |phi|)^2
-// (phase_field.f90:27-52: k+- are sums of squared liquid / solid fractions
-// of the taps), which FP32 phi itself loses below its 6e-8 spacing at 1:
-// after 4000 steps of a 64^2 cavity that stall left a 3.4e-5 error in the
-// liquid fraction.  phi stays in [-1, 1] (each update is a convex
-// combination of phi_prec and +-1 with weights k+-/2 <= 9/16), so the
-// sign bit decides the branch; the absolute values fold a rounding
-// overshoot past +-1 back by its own size instead of flipping the phase.
-template <typename R>
-__device__ __forceinline__ R phi_store(double phi) {
-  if constexpr (sizeof(R) == sizeof(double)) return phi;
-  else return phi >= 0.0 ? -(R)fabs(1.0 - phi) : (R)fabs(1.0 + phi);
-}
-template <typename R>
-__device__ __forceinline__ double phi_load(R e) {
-  if constexpr (sizeof(R) == sizeof(double)) return e;
-  else return (double)e + ((__float_as_uint(e) >> 31) ? 1.0 : -1.0);
-}
-
 // Planes of one moment state (typed by the kernels through R, EQF).
 struct MomIn {
   const double2* a;
@@ -116,7 +93,7 @@ struct Node {
   double2 a;                       // rho, T - T_crit
   typename Vec<R>::V4 b;           // u, v, Pf_xx, Pf_yy  (EQF: u, v, Pg_xx, Pg_yy)
   typename CType<R, EQF>::T c;     // Pf_xy, Pg_xx, Pg_yy, Pg_xy  (EQF: Pg_xy)
-  R phi;
+  double phi;
 };
 
 template <typename R, bool EQF>
@@ -332,23 +309,21 @@ __device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
 // rebuilt populations.
 struct NodeOut {
   double rho, Tav, uav, vav;
-  double phi;  // the stored (R) phi
 };
 
-// One launch of the moment-storage step: state and phi (R planes) in and
-// out, the east-wall temperature column of the previous and of this step,
-// an optional probe record (T averaged as moments.f90 at the nodes whose
+// One launch of the moment-storage step: state and phi in and out, the
+// east-wall temperature column of the previous and of this step, an
+// optional probe record (T averaged as moments.f90 at the nodes whose
 // probe_idx is >= 0, written to probe_out[probe_idx]; probe_idx in column 0
-// of a row is >= 0 when the row holds a probe) and optional macroscopic
-// planes rho, u, v, T to write instead of the handle's (their halo columns
-// of u, v must hold the ghost velocities; the handle's FP64 phi and
-// phi_prec planes are then left alone).
+// of a row is >= 0 when the row holds a probe) and optional
+// macroscopic planes rho, u, v, T to write instead of the handle's (their
+// halo columns of u, v must hold the ghost velocities).
 struct MomStep {
   double* macro[4];
   MomIn in;
   MomOut out;
-  const void* pa;
-  void* pb;
+  const double* pa;
+  double* pb;
   const double* te_in;
   double* te_out;
   bool write_all;

@@ -51,10 +51,8 @@ enum {
                        FP32; phase field, enthalpy update and moments in FP64  */
   BGK_FP32_MOM = 2, /* moment storage: per node rho and T - T_crit in FP64,
                        u, v and the non-equilibrium second moments of f and g
-                       (times 1 - omega) and phi in FP32, 52 bytes instead of
-                       152 (40 with tau_f = 1); populations are rebuilt from
-                       them at each pull; downloads of phi and phi_prec
-                       return the stored FP32 values.  Clean
+                       (times 1 - omega) in FP32, 48 bytes instead of 144;
+                       populations are rebuilt from them at each pull.  Clean
                        rounding of the collision (STEP_7's FP32 temporaries
                        ft, gt are not representable); no-slip cavity only;
                        populations are not addressable (download/upload of

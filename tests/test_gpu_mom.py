@@ -2,11 +2,8 @@
 oracle.
 
 The moment kernel rebuilds every pulled population from the upstream node's
-(rho, T) in FP64 and (u, v, Pf, Pg) in FP32, and keeps phi in FP32 as its
-distance to the nearer phase value, so it agrees with the clean FP64 policy
-up to the FP32 rounding of u, v, the non-equilibrium moments and phi.  (phi
-stored as plain FP32 fails the 4000-step bound below: 3.4e-5 in the liquid
-fraction, the slow approach of nodes to +-1 lost under FP32's spacing at 1.)
+(rho, T) in FP64 and (u, v, Pf, Pg) in FP32, so it agrees with the clean
+FP64 policy up to the FP32 rounding of u, v and the non-equilibrium moments.
 Error measure: relative max-norm max|a - b| / max|b| per field, with the
 liquid fraction (1+phi)/2 in place of phi; on the small grids u and v are
 both scaled by the largest velocity component (in a 1 x 1 cavity u is a

@@ -321,38 +321,3 @@ def test_buoyancy_reference_temperature():
     assert np.all(a > 0.0)
     np.testing.assert_allclose(a / b, (283.0 - 278.0) / (283.0 - 273.0), rtol=1e-14)
     np.testing.assert_allclose(dv(283.0, 0.0) / b, 283.0 / 10.0, rtol=1e-14)
-
-
-def test_fp32_phi_needs_the_phase_distance_encoding():
-    """Why the moment storage keeps phi as its distance to the nearer phase
-    value: phi rounded to FP32 after every step of the oracle (64^2 cavity,
-    4000 steps) drifts 3.4e-5 in liquid fraction from the unrounded run, over
-    the 1e-5 storage bound, because a node approaching +-1 moves by
-    increments of order (1 - |phi|)^2 that FP32's spacing at 1 swallows;
-    rounded as e = -(1 - phi) (phi >= 0) or 1 + phi, decoded by the sign
-    bit, it stays within 2e-6."""
-    p = Params(nx=64, ny=64, length=64.0, tau_f=1.0, rounding=1)
-
-    def plain(phi):
-        return phi.astype(np.float32).astype(np.float64)
-
-    def encoded(phi):
-        e = np.where(phi >= 0, -np.abs(1.0 - phi).astype(np.float32),
-                     np.abs(1.0 + phi).astype(np.float32))
-        return np.where(np.signbit(e), 1.0 + e.astype(np.float64), e.astype(np.float64) - 1.0)
-
-    ref, a, b = Oracle(p), Oracle(p), Oracle(p)
-    for _ in range(4000):
-        for o, rnd in ((ref, None), (a, plain), (b, encoded)):
-            o.step(1)
-            if rnd is not None:
-                f = o.field("phi")
-                f[...] = rnd(f)
-    fl = lambda o: (1.0 + o.field("phi")[1:-1, 1:-1]) / 2.0
-    err_plain = np.abs(fl(a) - fl(ref)).max()
-    err_enc = np.abs(fl(b) - fl(ref)).max()
-    assert err_plain > 1e-5, err_plain
-    assert err_enc < 2e-6, err_enc
-    # the encoding round-trips the phase values exactly
-    for v in (-1.0, 1.0, 0.0):
-        assert encoded(np.array([v]))[0] == v
