@@ -1,10 +1,13 @@
 """Throughput of the BGK_PCM time step on one MI355X.
 
-Workload: the melting cavity of the reference benchmark input
-UTIL/input.dat.0 (no-slip walls, tau_f=1, tau_g=0.51, Stefan 0.1) on a
-4096 x 4096 lattice with grav set for Ra = 1e7, advanced by the fused step
-with FP32 moment storage (per node rho and T - 273 in FP64; u, v and the
-thermal non-equilibrium moments in FP32; FP64 arithmetic).  One "step" is
+Workload: the 4096^2 FP32-storage config, a melting cavity with Ra = 1e7
+(grav set by the reference's definition), Stefan number 0.05 and a
+solid/liquid conductivity ratio of 2 (the k_ratio extension of the moment
+handles), otherwise the reference benchmark input UTIL/input.dat.0 (no-slip
+walls, tau_f=1, tau_g=0.51: the config's Pr 0.02 diverges at this Ra),
+advanced by the fused step with FP32 moment storage (per node rho and
+T - 273 in FP64; u, v and the thermal non-equilibrium moments in FP32; FP64
+arithmetic).  One "step" is
 one time step of the whole lattice (BC_f ... moments of the reference time
 loop).  Inputs are resident in HBM before the timed region.
 
@@ -16,11 +19,16 @@ Printed JSON line (one):
                  kernel's mean launch time, measured with HIP events on the
                  stream the kernel runs on; vs 8 TB/s
   fp32_population the fused step with FP32 population storage (f - w*rho0,
-                 g - w*T_crit; 160 B per update), same measurements
-  fp64           the FP64 fused kernel (304 B per update)
-  fp32_error     relative max-norm error (per field, liquid fraction for
-                 phi) after --error-steps (default 20000, the config's run
-                 length) of both FP32 storages against the FP64 run with
+                 g - w*T_crit; 160 B per update), same measurements, on the
+                 same cavity with conductivity ratio 1 (the population
+                 kernels carry no extensions)
+  fp64           the FP64 fused kernel (304 B per update), likewise
+  fp32_mom_error relative max-norm error (per field, liquid fraction for
+                 phi) of the FP32 moment run against the FP64 moment run
+                 with the same conductivity ratio after 1/4, 1/2 and all of
+                 --error-steps (default 20000, the config's run length)
+  fp32_error     with conductivity ratio 1, the errors after --error-steps
+                 of both FP32 storages against the FP64 population run with
                  clean FP64 temporaries and against the STEP_7-rounding FP64
                  run, and of the STEP_7 run against the clean one
   adjoint        gradient of a probe-temperature loss with respect to
@@ -98,6 +106,8 @@ def parse():
 
 
 RAYLEIGH = 1e7
+STEFAN = 0.05  # the 4096^2 FP32-storage config
+K_RATIO = 2.0
 
 
 def cavity(n: int):
@@ -130,9 +140,9 @@ def timed(lat, stream, steps: int, warmup: int):
     return ev0.elapsed_time(ev1) / steps, wall
 
 
-def gpu_run(p, stream, steps, warmup):
+def gpu_run(p, stream, steps, warmup, ext=None):
     from bgk_pcm_amd.lattice import Lattice
-    lat = Lattice(p)
+    lat = Lattice(p, **(ext or {}))
     lat.set_stream(stream.cuda_stream)
     kms, wms = timed(lat, stream, steps, warmup)
     fc = lat.field_check()
@@ -176,6 +186,28 @@ def fp32_error(p64, storages: dict, steps: int):
     return out
 
 
+def mom_error_vs_fp64_mom(p, ext: dict, error_steps: int):
+    """The FP32 moment run against the FP64 moment run with the same
+    physics extensions (the same model in FP64; the clean FP64 population
+    kernel agrees with it to 1e-10 at small sizes), both from the same
+    initial state, after 1/4, 1/2 and all of error_steps."""
+    from bgk_pcm_amd.lattice import Lattice
+    from bgk_pcm_amd.params import FP32_MOM, FP64_MOM
+    marks = sorted({max(error_steps // 4, 1), max(error_steps // 2, 1), error_steps})
+    errs = []
+    with Lattice(p.replace(precision=FP64_MOM), **ext) as l64, \
+            Lattice(p.replace(precision=FP32_MOM), **ext) as l32:
+        done = 0
+        for m in marks:
+            l64.step(m - done)
+            l32.step(m - done)
+            done = m
+            e = rel_errors({f: l32.download(f) for f in FIELDS},
+                           {f: l64.download(f) for f in FIELDS})
+            errs.append(dict(steps=m, **e))
+    return errs
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -189,7 +221,8 @@ def cpu_model() -> str:
 def cpu_baseline(p, steps: int):
     from bgk_pcm_amd.params import write_input_dat
     exe = ROOT / "oracle" / "_ref" / "openmp" / "ref_bench"
-    sample = f"{p.nx}x{p.ny} cavity, {steps} steps (same input as the GPU run), {cpu_model()}"
+    sample = (f"{p.nx}x{p.ny} cavity, {steps} steps (the GPU run's input.dat; the reference "
+              f"has no conductivity ratio), {cpu_model()}")
     if exe.exists():
         env = dict(os.environ)
         env.setdefault("OMP_PROC_BIND", "spread")
@@ -377,11 +410,11 @@ def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: 
     return res
 
 
-def measure(p, stream, steps, warmup, kind, kernel):
+def measure(p, stream, steps, warmup, kind, kernel, ext=None):
     """MLUPS from the wall clock and roofline figures of the fused kernel
-    from HIP events on its stream."""
+    from HIP events on its stream (ext: physics extensions of the handle)."""
     n = p.nx
-    kms, wms, _ = gpu_run(p, stream, steps, warmup)
+    kms, wms, _ = gpu_run(p, stream, steps, warmup, ext)
     lups = p.nx * p.ny
     nbytes = BYTES_PER_LUP[kind] * lups
     ach = nbytes / (kms * 1e-3) / 1e9
@@ -408,14 +441,17 @@ def main():
         return
     stream = torch.cuda.Stream()
     n = args.size
-    p = cavity(n)
+    # the config's Stefan number; its conductivity ratio is an extension that
+    # only the moment handles carry (the population kernels run ratio 1)
+    p = cavity(n).replace(stefan_number=STEFAN)
+    ext = {"k_ratio": K_RATIO}
     # the moment and FP32-population kernels follow the clean rounding of the
     # collision; beta (forces.f90:13) is then FP64 as well
     pm = p.replace(precision=FP32_MOM, rounding=ROUND_CLEAN)
     pp = p.replace(precision=FP32_DEV, rounding=ROUND_CLEAN)
     mom_kind = "mom_eq" if p.tau_f == 1.0 else "mom"
 
-    mm = measure(pm, stream, args.steps, args.warmup, mom_kind, "k_fused_mom")
+    mm = measure(pm, stream, args.steps, args.warmup, mom_kind, "k_fused_mom", ext)
     res = {
         "metric": METRIC,
         "value": mm["mlups"],
@@ -430,7 +466,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"melting cavity (UTIL/input.dat.0 physics, Ra 1e7), {n}x{n}, "
+            "workload": f"melting cavity {n}x{n}, Ra 1e7, Stefan {STEFAN}, solid/liquid "
+                        f"conductivity ratio {K_RATIO:g} (UTIL/input.dat.0 otherwise), "
                         "fused time step, FP32 moment storage",
             "nx": n, "ny": n,
             "storage": "per node rho, T-273 FP64; u, v, (1-omega_g) Pi_g^neq FP32"
@@ -438,7 +475,7 @@ def main():
             "arithmetic": "f64",
             "boundaries": "no-slip (type 4) on all walls",
             "tau_f": p.tau_f, "tau_g": p.tau_g, "grav": p.grav, "rayleigh": RAYLEIGH,
-            "stefan_number": p.stefan_number,
+            "stefan_number": p.stefan_number, "k_ratio": K_RATIO,
         },
         "roofline": {
             "bound": "hbm",
@@ -461,7 +498,10 @@ def main():
         res["fp32_population_over_fp64"] = round(
             res["fp32_population"]["mlups"] / res["fp64"]["mlups"], 3)
     if args.error_steps > 0:
-        progress(f"FP32 storage errors after {args.error_steps} steps")
+        progress(f"FP32 moment errors, conductivity ratio {K_RATIO:g}, {args.error_steps} steps")
+        res["fp32_mom_error"] = {"k_ratio": K_RATIO,
+                                 "vs_fp64_mom": mom_error_vs_fp64_mom(pm, ext, args.error_steps)}
+        progress(f"FP32 storage errors, conductivity ratio 1, {args.error_steps} steps")
         res["fp32_error"] = fp32_error(p, {"mom": pm, "fp32_population": pp}, args.error_steps)
     # one GPU's share of the 8192^2 lattice over 8 GPUs (design-only sharding)
     ps = cavity(8192).replace(ny=1026, precision=FP32_MOM, rounding=ROUND_CLEAN)
