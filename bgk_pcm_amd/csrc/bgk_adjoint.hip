@@ -87,7 +87,14 @@ struct AdjArgs {
 // neither stored nor read) and the state before step n (moments and phi).
 // Every read of a node's update lies in its 3 x 3 neighbourhood, so each
 // value comes from HBM once per tile instead of from L2 for every neighbour.
-constexpr int BX = 64, BY = 4, WX = BX + 2, WY = BY + 2;
+// Tiles of 64 x 8 nodes (77 KB of LDS with FP32 moments and omega_f = 1:
+// two workgroups of 8 waves per CU, 4 waves per SIMD at 110 VGPRs; 90-122 KB
+// and one workgroup for the other layouts).  The halo re-reads are 1.55x
+// the tile's nodes at 64 x 4, 1.29x at 64 x 8, 1.16x at 64 x 16.  Measured
+// at 8192^2 (FP32, omega_f = 1) per launch: 64 x 4 4.33 ms (3 workgroups
+// per CU), 64 x 8 3.66 ms, 64 x 16 4.29 ms (one workgroup per CU: no other
+// workgroup computes while it fills its window).
+constexpr int BX = 64, BY = 8, WX = BX + 2, WY = BY + 2;
 
 template <bool EQF>
 struct LSlots {
@@ -487,18 +494,23 @@ __global__ __launch_bounds__(BX* BY) void k_adjoint(AdjArgs a) {
     }
   }
   // ---- parameter adjoints: workgroup sum into this workgroup's own slot
-  __shared__ double red[3][BX * BY];
+  // (butterfly within each wave, then the waves in order: deterministic)
+  constexpr int NW = BX * BY / 64;
+  __shared__ double red[3][NW];
   const int t = threadIdx.y * BX + threadIdx.x;
-  for (int c = 0; c < 3; ++c) red[c][t] = g[c];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    for (int m = 32; m > 0; m >>= 1) g[c] += __shfl_xor(g[c], m, 64);
+  if ((t & 63) == 0)
+    for (int c = 0; c < 3; ++c) red[c][t >> 6] = g[c];
   __syncthreads();
-  for (int w = BX * BY / 2; w > 0; w >>= 1) {
-    if (t < w)
-      for (int c = 0; c < 3; ++c) red[c][t] += red[c][t + w];
-    __syncthreads();
-  }
   if (t == 0) {
     const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
-    for (int c = 0; c < 3; ++c) a.partial[3 * b + c] += red[c][0];
+    for (int c = 0; c < 3; ++c) {
+      double sum = 0.0;
+      for (int w = 0; w < NW; ++w) sum += red[c][w];
+      a.partial[3 * b + c] += sum;
+    }
   }
 }
 
