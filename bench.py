@@ -2,8 +2,7 @@
 
 Workload: the 4096^2 FP32-storage config, a melting cavity with Ra = 1e7
 (grav set by the reference's definition), Stefan number 0.05 and a
-solid/liquid conductivity ratio of 2 (the k_ratio extension of the moment
-handles), otherwise the reference benchmark input UTIL/input.dat.0 (no-slip
+solid/liquid conductivity ratio of 2 (the k_ratio extension), otherwise the reference benchmark input UTIL/input.dat.0 (no-slip
 walls, tau_f=1, tau_g=0.51: the config's Pr 0.02 diverges at this Ra),
 advanced by the fused step with FP32 moment storage (per node rho and
 T - 273 in FP64; u, v and the thermal non-equilibrium moments in FP32; FP64
@@ -19,18 +18,17 @@ Printed JSON line (one):
                  kernel's mean launch time, measured with HIP events on the
                  stream the kernel runs on; vs 8 TB/s
   fp32_population the fused step with FP32 population storage (f - w*rho0,
-                 g - w*T_crit; 160 B per update), same measurements, on the
-                 same cavity with conductivity ratio 1 (the population
-                 kernels carry no extensions)
+                 g - w*T_crit; 160 B per update), same workload and
+                 measurements
   fp64           the FP64 fused kernel (304 B per update), likewise
   fp32_mom_error relative max-norm error (per field, liquid fraction for
                  phi) of the FP32 moment run against the FP64 moment run
                  with the same conductivity ratio after 1/4, 1/2 and all of
                  --error-steps (default 20000, the config's run length)
-  fp32_error     with conductivity ratio 1, the errors after --error-steps
-                 of both FP32 storages against the FP64 population run with
-                 clean FP64 temporaries and against the STEP_7-rounding FP64
-                 run, and of the STEP_7 run against the clean one
+  fp32_error     the errors after --error-steps of both FP32 storages
+                 against the FP64 population run with clean FP64
+                 temporaries and against the STEP_7-rounding FP64 run, and
+                 of the STEP_7 run against the clean one
   adjoint        gradient of a probe-temperature loss with respect to
                  (stefan_number, k_ratio, t_hot) by the GPU adjoint on the
                  adjoint workload (512^2, FP64 moments, 2000 steps,
@@ -163,11 +161,12 @@ def rel_errors(a, b):
     return out
 
 
-def fp32_error(p64, storages: dict, steps: int):
+def fp32_error(p64, storages: dict, steps: int, ext=None):
     """Errors after `steps` steps from the same initial state: each FP32
     storage run against the FP64 run with clean FP64 temporaries and against
     the STEP_7-rounding FP64 run, and the STEP_7 run against the clean one
-    (the reference's own FP32-temporary noise floor)."""
+    (the reference's own FP32-temporary noise floor); ext: physics
+    extensions of every run."""
     from bgk_pcm_amd.lattice import Lattice
     from bgk_pcm_amd.params import FP64, ROUND_CLEAN, ROUND_STEP7
     assert p64.precision == FP64
@@ -175,7 +174,7 @@ def fp32_error(p64, storages: dict, steps: int):
     todo = [("fp64_step7", p64.replace(rounding=ROUND_STEP7)),
             ("fp64_clean", p64.replace(rounding=ROUND_CLEAN))] + list(storages.items())
     for name, p in todo:
-        with Lattice(p) as lat:
+        with Lattice(p, **(ext or {})) as lat:
             lat.step(steps)
             runs[name] = {f: lat.download(f) for f in FIELDS}
     out = {"steps": steps}
@@ -441,8 +440,8 @@ def main():
         return
     stream = torch.cuda.Stream()
     n = args.size
-    # the config's Stefan number; its conductivity ratio is an extension that
-    # only the moment handles carry (the population kernels run ratio 1)
+    # the config's Stefan number and conductivity ratio (an extension of
+    # every handle; the reference has none)
     p = cavity(n).replace(stefan_number=STEFAN)
     ext = {"k_ratio": K_RATIO}
     # the moment and FP32-population kernels follow the clean rounding of the
@@ -490,10 +489,11 @@ def main():
             "algorithmic_bytes_per_launch": mm["algorithmic_bytes_per_launch"],
         },
     }
-    res["fp32_population"] = measure(pp, stream, args.steps, args.warmup, "fp32", "k_fused32")
+    res["fp32_population"] = measure(pp, stream, args.steps, args.warmup, "fp32", "k_fused32",
+                                     ext)
     if not args.no_fp64:
         res["fp64"] = measure(p, stream, max(args.steps // 2, 1), args.warmup, "fp64",
-                              "k_fused64")
+                              "k_fused64", ext)
         res["mom_over_fp64"] = round(res["value"] / res["fp64"]["mlups"], 3)
         res["fp32_population_over_fp64"] = round(
             res["fp32_population"]["mlups"] / res["fp64"]["mlups"], 3)
@@ -501,8 +501,9 @@ def main():
         progress(f"FP32 moment errors, conductivity ratio {K_RATIO:g}, {args.error_steps} steps")
         res["fp32_mom_error"] = {"k_ratio": K_RATIO,
                                  "vs_fp64_mom": mom_error_vs_fp64_mom(pm, ext, args.error_steps)}
-        progress(f"FP32 storage errors, conductivity ratio 1, {args.error_steps} steps")
-        res["fp32_error"] = fp32_error(p, {"mom": pm, "fp32_population": pp}, args.error_steps)
+        progress(f"FP32 storage errors, {args.error_steps} steps")
+        res["fp32_error"] = fp32_error(p, {"mom": pm, "fp32_population": pp}, args.error_steps,
+                                       ext)
     # one GPU's share of the 8192^2 lattice over 8 GPUs (design-only sharding)
     ps = cavity(8192).replace(ny=1026, precision=FP32_MOM, rounding=ROUND_CLEAN)
     res["slab"] = dict(measure(ps, stream, args.steps, args.warmup, mom_kind, "k_fused_mom"),

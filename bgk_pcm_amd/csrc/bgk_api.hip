@@ -419,12 +419,14 @@ int bgk_set_extensions(bgk_handle* h, const bgk_ext* e) {
                 "need t_hot >= 0, k_ratio > 0, east_cold 0/1 with t_cold > 0, noise_amp >= 0, "
                 "t_crit >= 0, t_ref >= 0",
                 e->t_hot, e->k_ratio, e->east_cold, e->t_cold, e->noise_amp, e->t_crit, e->t_ref);
+  // every handle carries the conductivity ratio (one relaxation rate per
+  // node in the collision); the others need a moment handle
   const bool mom = h->prm.precision == BGK_FP32_MOM || h->prm.precision == BGK_FP64_MOM;
-  const bool off = (e->t_hot == 0.0 || e->t_hot == h->prm.t0) && e->k_ratio == 1.0 &&
-                   !e->east_cold && e->noise_amp == 0.0 &&
-                   (e->t_crit == 0.0 || e->t_crit == 273.0) && e->t_ref == 0.0;
+  const bool off = (e->t_hot == 0.0 || e->t_hot == h->prm.t0) && !e->east_cold &&
+                   e->noise_amp == 0.0 && (e->t_crit == 0.0 || e->t_crit == 273.0) &&
+                   e->t_ref == 0.0;
   if (!mom && !off)
-    return fail(BGK_E_UNSUPPORTED, "physics extensions need a moment handle "
+    return fail(BGK_E_UNSUPPORTED, "physics extensions other than k_ratio need a moment handle "
                                    "(precision BGK_FP32_MOM or BGK_FP64_MOM)");
   h->ext = *e;
   set_constants(h);

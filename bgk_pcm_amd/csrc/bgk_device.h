@@ -118,6 +118,18 @@ struct Consts {
   double t_ref;        // buoyancy reference temperature (0: STEP_7's absolute T)
 };
 
+// Thermal relaxation rate of a node whose phase field (after phase_field)
+// is phi: c.omega_g, or with the conductivity-ratio extension alpha(phi) =
+// alpha_l (fl + k_ratio (1 - fl)), fl = (1 + phi)/2, tau_g = 1/2 +
+// alpha/cs2, in the oracle's order of operations so that the FP64 kernels
+// stay bit for bit equal to it.
+__device__ __forceinline__ double omega_g_at(const Consts& c, double phi) {
+  if (c.k_ratio == 1.0) return c.omega_g;
+  const double fl = 0.5 * (1.0 + phi);
+  const double alpha = c.alpha_l * (fl + c.k_ratio * (1.0 - fl));
+  return 1.0 / (0.5 + alpha / c.cs2);
+}
+
 // Uniform [0,1) number of node q for the initial noise: SplitMix64 output
 // for the state seed + (q+1)*0x9E3779B97F4A7C15 (Steele, Lea and Flood's
 // generator with Vigna's constants), so that every node's value is

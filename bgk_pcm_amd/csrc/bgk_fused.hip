@@ -130,7 +130,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pairs64 A, Pairs64 B,
   // ---- collision (STEP_7/collision.f90:14-150)
   double fo[9], go[9];
   regularised_bgk<STEP7>(c, c.omega_f, rho, u, v, fp, fo);
-  regularised_bgk<STEP7>(c, c.omega_g, T, u, v, gp, go);
+  regularised_bgk<STEP7>(c, omega_g_at(c, phi), T, u, v, gp, go);
 #pragma unroll
   for (int k = 0; k < 9; ++k) stream_store(&B.p[k][q], fo[k], go[k]);
   stream_store(&pb[q], phi);

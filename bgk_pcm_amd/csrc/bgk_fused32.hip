@@ -183,7 +183,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
     dgr[k] = (RC)dg[k];
   }
   regularised_dev<RC>(c, (RC)c.omega_f, (RC)drho, (RC)rho, (RC)u, (RC)v, dfr, fo);
-  regularised_dev<RC>(c, (RC)c.omega_g, (RC)dT, (RC)T, (RC)u, (RC)v, dgr, go);
+  regularised_dev<RC>(c, (RC)omega_g_at(c, phi), (RC)dT, (RC)T, (RC)u, (RC)v, dgr, go);
   float fs[9], gs[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {

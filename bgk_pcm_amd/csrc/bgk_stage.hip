@@ -321,8 +321,8 @@ __global__ void k_forces(Macro m, const double* phi, const double* pp, Consts c,
 // ------------------------------------------------------------------ collision
 // STEP_7/collision.f90:14-152.
 template <bool STEP7>
-__global__ void k_collision(Pops64 fp, Pops64 gp, Pops64 f, Pops64 g, Macro m, Consts c,
-                            Geom gm) {
+__global__ void k_collision(Pops64 fp, Pops64 gp, Pops64 f, Pops64 g, Macro m,
+                            const double* __restrict__ phi, Consts c, Geom gm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   if (i > gm.nx || j > gm.ny) return;
@@ -336,7 +336,7 @@ __global__ void k_collision(Pops64 fp, Pops64 gp, Pops64 f, Pops64 g, Macro m, C
   for (int k = 0; k < 9; ++k) f.p[k][q] = out[k];
 #pragma unroll
   for (int k = 0; k < 9; ++k) pop[k] = gp.p[k][q];
-  regularised_bgk<STEP7>(c, c.omega_g, m.T[q], u, v, pop, out);
+  regularised_bgk<STEP7>(c, omega_g_at(c, phi[q]), m.T[q], u, v, pop, out);
 #pragma unroll
   for (int k = 0; k < 9; ++k) g.p[k][q] = out[k];
 }
@@ -433,11 +433,11 @@ hipError_t launch_collision(bgk_handle* h) {
   if (h->prm.rounding == BGK_ROUND_STEP7)
     hipLaunchKernelGGL(k_collision<true>, interior_grid(h->geom), dim3(TX, TY), 0, h->stream,
                        pops(h->f[1 - h->cur]), pops(h->g[1 - h->cur]), pops(h->f[h->cur]),
-                       pops(h->g[h->cur]), macro_of(h), h->c, h->geom);
+                       pops(h->g[h->cur]), macro_of(h), h->phi[h->pc], h->c, h->geom);
   else
     hipLaunchKernelGGL(k_collision<false>, interior_grid(h->geom), dim3(TX, TY), 0, h->stream,
                        pops(h->f[1 - h->cur]), pops(h->g[1 - h->cur]), pops(h->f[h->cur]),
-                       pops(h->g[h->cur]), macro_of(h), h->c, h->geom);
+                       pops(h->g[h->cur]), macro_of(h), h->phi[h->pc], h->c, h->geom);
   return hipGetLastError();
 }
 

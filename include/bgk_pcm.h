@@ -136,8 +136,10 @@ typedef struct bgk_info {
 
 /* Physics extensions (not in the reference; the defaults t_hot = 0 ->
  * T0, k_ratio = 1, east_cold = 0, noise_amp = 0, t_crit = 0 -> 273 and
- * t_ref = 0 give STEP_7's step).  Honoured by moment handles
- * (BGK_FP32_MOM, BGK_FP64_MOM); other handles accept only the defaults. */
+ * t_ref = 0 give STEP_7's step).  k_ratio is honoured by every handle
+ * (fused and per-subroutine collisions); the others by moment handles
+ * (BGK_FP32_MOM, BGK_FP64_MOM) only, and other handles refuse them with
+ * BGK_E_UNSUPPORTED. */
 typedef struct bgk_ext {
   double t_hot;       /* hot (west) wall and initial liquid temperature in
                          place of T0 (0: T0); the scale T0 - T_in_sol of

@@ -53,6 +53,21 @@ def test_fp32_storage_error_bound(oracles, rounding):
     assert max(err.values()) <= 1e-5, err
 
 
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_fp32_storage_with_conductivity_ratio(rounding):
+    """The conductivity-ratio extension on FP32 population storage, against
+    the oracle with the same ratio (64x64, 4000 steps, bound 1e-5)."""
+    base = Params(nx=64, ny=64, length=64.0, rounding=rounding)
+    print(f"fp32 storage, k_ratio 2, 64x64, 4000 steps vs FP64 rounding={rounding}", flush=True)
+    o = Oracle(base, k_ratio=2.0)
+    o.step(4000)
+    with Lattice(base.replace(precision=FP32_DEV), k_ratio=2.0) as lat:
+        lat.step(4000)
+        err = rel_errors(lat, o)
+    print("relative max-norm errors:", err)
+    assert max(err.values()) <= 1e-5, err
+
+
 def test_fp32_collision_arithmetic_error_bound(oracles):
     base, res = oracles
     with Lattice(base.replace(precision=FP32_DEV, fp32_collision=1)) as lat:

@@ -182,8 +182,10 @@ def test_critical_and_reference_temperatures_match_oracle(precision, tol, t_crit
 
 
 def test_extensions_need_a_moment_handle():
+    """All but the conductivity ratio, which every handle carries."""
     with Lattice(Params(nx=8, ny=8)) as lat:
         lat.set_extensions(0.0, 1.0)  # the defaults are accepted everywhere
+        lat.set_extensions(0.0, 2.0)
         with pytest.raises(BgkError) as e:
             lat.set_extensions(284.0, 1.0)
         assert e.value.code == -4

@@ -111,6 +111,39 @@ def test_fused_vs_oracle(nx, ny, nsteps, rounding):
     assert compare_interior(got, want, POPS) == []
 
 
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_conductivity_ratio_is_bit_exact_on_both_fp64_paths(rounding):
+    """The conductivity-ratio extension (a thermal relaxation rate per node
+    from its phi after phase_field) on an FP64 handle: the fused kernel and
+    the per-subroutine kernels against the oracle with the same ratio, bit
+    for bit; the ratio does change the result."""
+    p = Params(nx=67, ny=45, rounding=rounding, re=3.0, length=45.0)
+    kr = 1.7
+    print(f"k_ratio {kr}, 67x45, rounding={rounding}: fused 150 steps", flush=True)
+    o = Oracle(p, k_ratio=kr)
+    o.step(150)
+    want = {f: o.field(f).copy() for f in STATE + POPS}
+    with Lattice(p, k_ratio=kr) as lat:
+        lat.step(150)
+        got = gpu_state(lat, STATE + POPS)
+    assert compare_state(got, want, STATE) == []
+    assert compare_interior(got, want, POPS) == []
+    o1 = Oracle(p)
+    o1.step(150)
+    assert not np.array_equal(o1.field("T"), want["T"])
+    print(f"k_ratio {kr}: per-subroutine 40 steps", flush=True)
+    o = Oracle(p, k_ratio=kr)
+    o.step(40)
+    with Lattice(p, k_ratio=kr) as lat:
+        for _ in range(40):
+            for s in Lattice.STAGES:
+                lat.stage(s)
+        got = gpu_state(lat, STATE + POPS)
+    want = {f: o.field(f) for f in STATE + POPS}
+    assert compare_state(got, want, STATE) == []
+    assert compare_interior(got, want, POPS) == []
+
+
 def test_any_split_of_steps_and_mixing_paths():
     p = Params(nx=40, ny=36, re=2.0, length=36.0)
     o = Oracle(p)
