@@ -88,7 +88,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--error-steps", type=int, default=20000)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
