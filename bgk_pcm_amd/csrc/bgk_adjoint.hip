@@ -87,13 +87,15 @@ struct AdjArgs {
 // neither stored nor read) and the state before step n (moments and phi).
 // Every read of a node's update lies in its 3 x 3 neighbourhood, so each
 // value comes from HBM once per tile instead of from L2 for every neighbour.
-// Tiles of 64 x 8 nodes (77 KB of LDS with FP32 moments and omega_f = 1:
-// two workgroups of 8 waves per CU, 4 waves per SIMD at 110 VGPRs; 90-122 KB
-// and one workgroup for the other layouts).  The halo re-reads are 1.55x
-// the tile's nodes at 64 x 4, 1.29x at 64 x 8, 1.16x at 64 x 16.  Measured
-// at 8192^2 (FP32, omega_f = 1) per launch: 64 x 4 4.33 ms (3 workgroups
-// per CU), 64 x 8 3.66 ms, 64 x 16 4.29 ms (one workgroup per CU: no other
-// workgroup computes while it fills its window).
+// Tiles of 64 x 8 nodes: with FP32 moments 53 KB of LDS (omega_f = 1) or
+// 69 KB, two workgroups of 8 waves per CU, 4 waves per SIMD at 110 VGPRs;
+// with FP64 moments 90 or 122 KB and one workgroup.  The halo re-reads are
+// 1.55x the tile's nodes at 64 x 4, 1.29x at 64 x 8, 1.16x at 64 x 16.
+// Measured at 8192^2 (FP32, omega_f = 1) per launch: 64 x 4 4.33 ms (3
+// workgroups per CU), 64 x 8 3.66 ms, 64 x 16 4.29 ms (one workgroup per
+// CU: no other workgroup computes while it fills its window).  The adjoint
+// planes sit in LDS in their stored precision (77 -> 53 KB: same time, and
+// two workgroups per CU for the general FP32 layout instead of one).
 constexpr int BX = 64, BY = 8, WX = BX + 2, WY = BY + 2;
 
 template <bool EQF>
@@ -107,7 +109,7 @@ struct LSlots {
 
 template <typename R, bool EQF>
 struct Win {
-  double lam[LSlots<EQF>::N][WY][WX];
+  R lam[LSlots<EQF>::N][WY][WX];  // as stored: every read widens exactly
   double2 a[WY][WX];
   typename Vec<R>::V4 b[WY][WX];
   typename CType<R, EQF>::T c[WY][WX];
