@@ -20,6 +20,15 @@ module bgk_gpu
      integer(c_int32_t) :: precision, rounding, device, fp32_collision
   end type bgk_params_t
 
+  ! physics extensions (bgk_ext; zeros and k_ratio = 1 keep STEP_7)
+  type, bind(C) :: bgk_ext_t
+     real(c_double) :: t_hot, k_ratio
+     integer(c_int32_t) :: east_cold, reserved
+     real(c_double) :: t_cold, noise_amp
+     integer(c_int64_t) :: noise_seed
+     real(c_double) :: t_crit, t_ref
+  end type bgk_ext_t
+
   type, bind(C) :: bgk_info_t
      integer(c_int64_t) :: device_bytes, device_free, device_total
      real(c_double) :: host_maxrss_mb
@@ -124,6 +133,11 @@ module bgk_gpu
        integer(c_int32_t), intent(in) :: i(*), j(*)
        real(c_double), intent(out) :: out(*)
      end function
+     integer(c_int) function c_bgk_set_extensions(h, e) bind(C, name='bgk_set_extensions')
+       import :: c_ptr, c_int, bgk_ext_t
+       type(c_ptr), value :: h
+       type(bgk_ext_t), intent(in) :: e
+     end function
      integer(c_int) function c_bgk_get_info(h, info) bind(C, name='bgk_get_info')
        import :: c_ptr, c_int, bgk_info_t
        type(c_ptr), value :: h
@@ -193,6 +207,31 @@ contains
     alpha = (1.d0/3.d0) * (prm%tau_g - 0.5d0)
     tin = prm%t0
   end subroutine gpu_setup
+
+  ! Physics extensions from a namelist file, before the initialization:
+  !   &bgk_ext t_hot = 283.5, k_ratio = 2.0, t_cold = 273.0,
+  !            noise_amp = 1.0d-3, noise_seed = 7, t_crit = 273.0, t_ref = 0 /
+  ! (any subset; t_cold > 0 puts the east wall at t_cold).  t_hot also
+  ! becomes the west-wall temperature of the diagnostics.
+  subroutine set_extensions(path)
+    character(len=*), intent(in) :: path
+    real(8) :: t_hot, k_ratio, t_cold, noise_amp, t_crit, t_ref
+    integer(8) :: noise_seed
+    type(bgk_ext_t) :: e
+    integer :: u
+    namelist /bgk_ext/ t_hot, k_ratio, t_cold, noise_amp, noise_seed, t_crit, t_ref
+    t_hot = 0.d0; k_ratio = 1.d0; t_cold = 0.d0; noise_amp = 0.d0; noise_seed = 0
+    t_crit = 0.d0; t_ref = 0.d0
+    open(newunit=u, file=path, status='old', action='read')
+    read(u, nml=bgk_ext)
+    close(u)
+    e%t_hot = t_hot; e%k_ratio = k_ratio
+    e%east_cold = merge(1, 0, t_cold > 0.d0); e%reserved = 0; e%t_cold = t_cold
+    e%noise_amp = noise_amp; e%noise_seed = noise_seed
+    e%t_crit = t_crit; e%t_ref = t_ref
+    call check(c_bgk_set_extensions(handle, e), 'bgk_set_extensions')
+    if (t_hot > 0.d0) tin = t_hot
+  end subroutine set_extensions
 
   subroutine gpu_finish()
     if (c_associated(handle)) call check(c_bgk_destroy(handle), 'bgk_destroy')

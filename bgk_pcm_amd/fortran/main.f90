@@ -21,6 +21,10 @@
 !              Outrho_, OutTem_, Outvel_) instead of the ASCII Output_ file
 !   --dump-out write dumpFile.dat at the end of the run (restart: dump = 1
 !              in input.dat reads it instead of the initialization)
+!   --ext FILE physics extensions from the namelist &bgk_ext in FILE
+!              (hot-wall temperature, conductivity ratio, cold east wall,
+!              initial noise, critical and buoyancy reference temperatures;
+!              all but the conductivity ratio need --mom)
 program bgk_thermal_gpu
   use iso_c_binding
   use bgk_gpu
@@ -31,7 +35,7 @@ program bgk_thermal_gpu
   integer :: icountT0, icountT1, icount_rate, icount_max, icountL0, icountL1
   real(8) :: time_inn_loop, mem_mb, dev_mb
   logical :: per_stage, bin_vtk, dump_at_end
-  character(len=64) :: arg
+  character(len=256) :: arg, ext_file
 
   write(6,*) "===================================="
   write(6,*) " MI355X GPU version (bgk_pcm_amd)  "
@@ -41,10 +45,20 @@ program bgk_thermal_gpu
   per_stage = .false.
   bin_vtk = .false.
   dump_at_end = .false.
+  ext_file = ''
   nargs = command_argument_count()
-  do k = 1, nargs
+  k = 0
+  do while (k < nargs)
+     k = k + 1
      call get_command_argument(k, arg)
      select case (trim(arg))
+     case ('--ext')
+        if (k == nargs) then
+           write(0,*) 'bgk_thermal_gpu: --ext needs a file name'
+           stop 2
+        end if
+        k = k + 1
+        call get_command_argument(k, ext_file)
      case ('--stages')
         per_stage = .true.
      case ('--fp32')
@@ -64,6 +78,7 @@ program bgk_thermal_gpu
      end select
   end do
   call gpu_setup(p)
+  if (len_trim(ext_file) > 0) call set_extensions(trim(ext_file))
   iter = p%iter
   isignal = 500
 
