@@ -288,7 +288,7 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S
   // where the forward uses the constant omega_g (the same value)
   {
     const double fl = 0.5 * (1.0 + o.phi);
-    const double domg_dalpha = -o.omega_g * o.omega_g / kCs2;
+    const double domg_dalpha = -o.omega_g * o.omega_g * 3.0;  // 1/cs2
     lphi += lomg * domg_dalpha * c.alpha_l * (1.0 - c.k_ratio) * 0.5;
     g[1] += lomg * domg_dalpha * c.alpha_l * (1.0 - fl);
   }

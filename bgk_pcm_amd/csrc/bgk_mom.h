@@ -245,20 +245,24 @@ struct Core {
   double qgxx, qgyy, qgxy;             // Pi_g^neq
 };
 
-// The reciprocal is the hardware estimate refined by two Newton steps
-// (r <- r + r (1 - d r), each doubling the correct bits), within an ulp or
-// two of the IEEE quotient and some twenty instructions cheaper than the
-// two divisions of the plain form (1.2% of the moment step at 4096^2);
-// the FP64 population kernels keep IEEE division to stay bit-exact.
-__device__ __forceinline__ double omega_g_of(const Consts& c, double phi) {
-  if (c.k_ratio == 1.0) return c.omega_g;
-  const double fl = 0.5 * (1.0 + phi);
-  const double alpha = c.alpha_l * (fl + c.k_ratio * (1.0 - fl));
-  const double d = 0.5 + alpha * 3.0;  // 1/2 + alpha/cs2
+// 1/d as the hardware estimate refined by two Newton steps (r <- r + r (1 -
+// d r), each doubling the correct bits): within an ulp or two of the IEEE
+// quotient and about ten instructions cheaper.  The moment path uses it for
+// its per-node reciprocals (the two divisions of the conductivity ratio's
+// rate cost 1.2% of the moment step at 4096^2); the FP64 population kernels
+// keep IEEE division to stay bit-exact.
+__device__ __forceinline__ double rcp_nr(double d) {
   double r = __builtin_amdgcn_rcp(d);
   r = fma(r, fma(-d, r, 1.0), r);
   r = fma(r, fma(-d, r, 1.0), r);
   return r;
+}
+
+__device__ __forceinline__ double omega_g_of(const Consts& c, double phi) {
+  if (c.k_ratio == 1.0) return c.omega_g;
+  const double fl = 0.5 * (1.0 + phi);
+  const double alpha = c.alpha_l * (fl + c.k_ratio * (1.0 - fl));
+  return rcp_nr(0.5 + alpha * 3.0);  // 1/(1/2 + alpha/cs2)
 }
 
 __device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
@@ -266,7 +270,7 @@ __device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
   Core o;
   o.rho = S.rho;
   o.dT2 = S.dT;
-  o.irho = 1.0 / o.rho;
+  o.irho = rcp_nr(o.rho);
   o.u2 = S.jx * o.irho;
   o.v2 = S.jy * o.irho;
 
