@@ -87,16 +87,22 @@ struct AdjArgs {
 // neither stored nor read) and the state before step n (moments and phi).
 // Every read of a node's update lies in its 3 x 3 neighbourhood, so each
 // value comes from HBM once per tile instead of from L2 for every neighbour.
-// Tiles of 64 x 8 nodes: with FP32 moments 53 KB of LDS (omega_f = 1) or
-// 69 KB, two workgroups of 8 waves per CU, 4 waves per SIMD at 110 VGPRs;
-// with FP64 moments 90 or 122 KB and one workgroup.  The halo re-reads are
-// 1.55x the tile's nodes at 64 x 4, 1.29x at 64 x 8, 1.16x at 64 x 16.
-// Measured at 8192^2 (FP32, omega_f = 1) per launch: 64 x 4 4.33 ms (3
-// workgroups per CU), 64 x 8 3.66 ms, 64 x 16 4.29 ms (one workgroup per
-// CU: no other workgroup computes while it fills its window).  The adjoint
-// planes sit in LDS in their stored precision (77 -> 53 KB: same time, and
-// two workgroups per CU for the general FP32 layout instead of one).
-constexpr int BX = 64, BY = 8, WX = BX + 2, WY = BY + 2;
+// Tiles of 64 x 4 nodes (one workgroup of 4 waves) or 64 x 8 (8 waves),
+// whichever keeps more waves per SIMD.  The window holds 80 B per node with
+// FP32 moments and omega_f = 1 (32 KB for 64 x 4: five workgroups per CU,
+// 5 waves per SIMD once the registers fit in 102, which the launch bound
+// asks for: 96 VGPRs, 6 spilled), 104 B with the general FP32 layout (8
+// waves per 69 KB at 64 x 8: 4 per SIMD), 136 B and 168 B with FP64 moments
+// (64 x 4: 3 and 2 waves per SIMD).  Measured at 8192^2 (FP32, omega_f = 1)
+// per launch: 64 x 4 3.26 ms at 5 waves per SIMD, 4.33 ms at 3 (the planes
+// then held in FP64 in LDS); 64 x 8 3.66 ms (4 waves); 64 x 16 4.29 ms (one
+// workgroup per CU, so none computes while it fills its window).  The halo
+// re-reads (1.55x the tile's nodes at 64 x 4, 1.29x at 64 x 8) are L2 hits.
+constexpr int BX = 64, WX = BX + 2;
+template <typename R, bool EQF>
+constexpr int tile_rows() {
+  return (sizeof(R) == sizeof(float) && !EQF) ? 8 : 4;
+}
 
 template <bool EQF>
 struct LSlots {
@@ -109,6 +115,7 @@ struct LSlots {
 
 template <typename R, bool EQF>
 struct Win {
+  static constexpr int WY = tile_rows<R, EQF>() + 2;
   R lam[LSlots<EQF>::N][WY][WX];  // as stored: every read widens exactly
   double2 a[WY][WX];
   typename Vec<R>::V4 b[WY][WX];
@@ -343,7 +350,8 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S
 }
 
 template <typename R, bool EQF>
-__global__ __launch_bounds__(BX* BY) void k_adjoint(AdjArgs a) {
+__global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjArgs a) {
+  constexpr int BY = tile_rows<R, EQF>(), WY = BY + 2;
   const int nx = a.gm.nx, ny = a.gm.ny;
   const int64_t P = a.gm.pitch;
   const int i0 = blockIdx.x * BX + 1, j0 = blockIdx.y * BY + 1;
@@ -591,6 +599,12 @@ size_t c_bytes(const bgk_handle* h) {
   if (h->mom_eqf) return f64 ? sizeof(double) : sizeof(float);
   return f64 ? sizeof(double4) : sizeof(float4);
 }
+// rows of the adjoint kernel's tile for the handle's instantiation
+int tile_rows_of(const bgk_handle* h) {
+  const bool f64 = h->prm.precision == BGK_FP64_MOM;
+  if (h->mom_eqf) return f64 ? tile_rows<double, true>() : tile_rows<float, true>();
+  return f64 ? tile_rows<double, false>() : tile_rows<float, false>();
+}
 
 struct Bump {
   char* base;
@@ -681,16 +695,17 @@ hipError_t launch_adjoint(bgk_handle* h, AdjCtx* x, const StatePtrs& sn, const S
   a.c = h->c;
   a.gm = h->geom;
   a.interf = h->nx / 40;
-  dim3 grd((unsigned)((h->nx + BX - 1) / BX), (unsigned)((h->ny + BY - 1) / BY));
+  const int by = tile_rows_of(h);
+  dim3 grd((unsigned)((h->nx + BX - 1) / BX), (unsigned)((h->ny + by - 1) / by));
   const bool f64 = h->prm.precision == BGK_FP64_MOM;
   if (h->mom_eqf && f64)
-    hipLaunchKernelGGL((k_adjoint<double, true>), grd, dim3(BX, BY), 0, h->stream, a);
+    hipLaunchKernelGGL((k_adjoint<double, true>), grd, dim3(BX, by), 0, h->stream, a);
   else if (h->mom_eqf)
-    hipLaunchKernelGGL((k_adjoint<float, true>), grd, dim3(BX, BY), 0, h->stream, a);
+    hipLaunchKernelGGL((k_adjoint<float, true>), grd, dim3(BX, by), 0, h->stream, a);
   else if (f64)
-    hipLaunchKernelGGL((k_adjoint<double, false>), grd, dim3(BX, BY), 0, h->stream, a);
+    hipLaunchKernelGGL((k_adjoint<double, false>), grd, dim3(BX, by), 0, h->stream, a);
   else
-    hipLaunchKernelGGL((k_adjoint<float, false>), grd, dim3(BX, BY), 0, h->stream, a);
+    hipLaunchKernelGGL((k_adjoint<float, false>), grd, dim3(BX, by), 0, h->stream, a);
   return hipGetLastError();
 }
 
@@ -733,7 +748,8 @@ hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg, int np,
   x->nckpt = (nsteps + lvl1 - 1) / lvl1;
   // (the first checkpoint of an outer segment is the outer checkpoint)
   x->nckpt2 = seg_outer > 0 ? (std::min(seg_outer, nsteps) + seg - 1) / seg - 1 : 0;
-  x->nblk = (int)(((h->nx + BX - 1) / BX) * ((h->ny + BY - 1) / BY));
+  const int by = tile_rows_of(h);
+  x->nblk = (int)(((h->nx + BX - 1) / BX) * ((h->ny + by - 1) / by));
   x->bytes = carve_ctx(h, x, nullptr);
   hipError_t e = hipMalloc(&x->pool, x->bytes);
   if (e != hipSuccess) {
