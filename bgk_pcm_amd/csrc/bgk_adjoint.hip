@@ -467,8 +467,10 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
       lphi += plane_at<R>(a.next, L_PC, q);
     }
     if (a.core) {
-      // ---- loss seed at a probe node (averaged T after step n)
-      if (a.seeds) {
+      // ---- loss seed at a probe node (averaged T after step n); column 0
+      // of the index plane flags the rows that hold probes, so the other
+      // rows skip the per-node index load
+      if (a.seeds && a.probe_idx[(int64_t)j * P] >= 0) {
         const int p = a.probe_idx[q];
         if (p >= 0) lTrep += a.seeds[p];
       }
