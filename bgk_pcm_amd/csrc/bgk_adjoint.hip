@@ -402,12 +402,6 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
           const bool frozen = (k == 6 && si == nx + 1 && sj == 0) ||
                               (k == 7 && si == nx + 1 && sj == ny + 1);
           if (!frozen) rebuild_adjoint(OPP[k], m, lam_f(W, k, ox, oy), 0.0, &lm);
-          // west ghost: equilibrium at the hot-wall temperature
-          if (si == 0) {
-            const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
-            const int64_t h = (int64_t)sjc * P;
-            g[2] += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(W, k, ox, oy);
-          }
         }
       }
       // ghost rows of g: row 0 is a copy of row 1, row Ny+1 of row Ny
@@ -500,7 +494,21 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
       plane_set<R>(v, L_KP, q, lkp);
       plane_set<R>(v, L_KM, q, lkm);
       plane_set<R>(v, L_PC, q, lpc);
-    } else if (i <= a.interf) {
+    }
+    // this node's pulls from the west ghost (column 1): equilibria at the
+    // hot-wall temperature (left for last: no register holds the sum while
+    // the rest of the node is reversed)
+    if (a.has_next && i == 1) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        if (CX[k] == 1) {
+          const int sj = j - CY[k];
+          const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
+          const int64_t h = (int64_t)sjc * P;
+          g[2] += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(W, k, ox, oy);
+        }
+    }
+    if (!a.core && i <= a.interf) {
       // initial state: T = t_hot in the liquid strip (initialization.f90:15-26)
       g[2] += lm.th;
     }
