@@ -91,10 +91,10 @@ struct AdjArgs {
 // whichever keeps more waves per SIMD.  The window holds 80 B per node with
 // FP32 moments and omega_f = 1 (32 KB for 64 x 4: five workgroups per CU,
 // 5 waves per SIMD once the registers fit in 102, which the launch bound
-// asks for: 96 VGPRs, 6 spilled), 104 B with the general FP32 layout (8
+// asks for: 96 VGPRs, 3 spilled), 104 B with the general FP32 layout (8
 // waves per 69 KB at 64 x 8: 4 per SIMD), 136 B and 168 B with FP64 moments
 // (64 x 4: 3 and 2 waves per SIMD).  Measured at 8192^2 (FP32, omega_f = 1)
-// per launch: 64 x 4 3.26 ms at 5 waves per SIMD, 4.33 ms at 3 (the planes
+// per launch: 64 x 4 3.18 ms at 5 waves per SIMD, 4.33 ms at 3 (the planes
 // then held in FP64 in LDS); 64 x 8 3.66 ms (4 waves); 64 x 16 4.29 ms (one
 // workgroup per CU, so none computes while it fills its window).  The halo
 // re-reads (1.55x the tile's nodes at 64 x 4, 1.29x at 64 x 8) are L2 hits.
