@@ -75,3 +75,13 @@ def test_invalid_parameters_are_rejected_before_touching_the_device():
         assert lib.bgk_create(ctypes.byref(c), ctypes.byref(h)) == -1
         assert lib.bgk_last_error()
     assert lib.bgk_step(None, 1) == -1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The time step runs on one GPU: bench.py --gpus 2 exits with status 2
+    and says so, before touching a device."""
+    root = pathlib.Path(__file__).resolve().parents[1]
+    r = subprocess.run(["python3", str(root / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
