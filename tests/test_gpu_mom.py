@@ -167,6 +167,20 @@ def test_extensions_match_oracle(precision, tol, t_hot, k_ratio):
 
 
 @pytest.mark.parametrize("precision,tol", [(FP64_MOM, 1e-10), (FP32_MOM, 1e-6)])
+def test_conductivity_ratio_general_layout_matches_oracle(precision, tol):
+    """The conductivity ratio with tau_f = 0.8 (the 48-byte layout: flow
+    non-equilibrium moments stored), 300 steps on 48 x 40."""
+    p = Params(nx=48, ny=40, length=48.0, grav=1e-3, rounding=ROUND_CLEAN, tau_f=0.8)
+    o = Oracle(p, k_ratio=2.0)
+    o.step(300)
+    with Lattice(p.replace(precision=precision), k_ratio=2.0) as lat:
+        lat.step(300)
+        err = rel_errors(lat, o, interior=False, speed_scale=True)
+    print(f"k_ratio 2, tau_f 0.8: {err}")
+    assert max(err.values()) <= tol, err
+
+
+@pytest.mark.parametrize("precision,tol", [(FP64_MOM, 1e-10), (FP32_MOM, 1e-6)])
 @pytest.mark.parametrize("t_crit,t_ref", [(275.0, 0.0), (273.0, 278.0), (271.5, 273.0)])
 def test_critical_and_reference_temperatures_match_oracle(precision, tol, t_crit, t_ref):
     """The phase field's T_crit and the buoyancy's reference temperature
