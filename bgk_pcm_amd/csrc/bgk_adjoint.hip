@@ -787,21 +787,29 @@ hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg, int np,
                          x->probe_idx, x->probe_ij, np, h->pitch);
     e = hipGetLastError();
   }
-  // halo velocities for recomputed steps (static ghost columns of u, v)
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(x->macro[1], h->u, (size_t)h->plane_elems * sizeof(double),
-                       hipMemcpyDeviceToDevice, h->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(x->macro[2], h->v, (size_t)h->plane_elems * sizeof(double),
-                       hipMemcpyDeviceToDevice, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   return e;
 }
 
+bool adj_forward_done(const bgk_handle* h) { return h->adj && h->adj->forward_done; }
+
+void adj_invalidate(bgk_handle* h) {
+  if (h->adj) h->adj->forward_done = false;
+}
+
 hipError_t adj_forward(bgk_handle* h, double* probe_T_host) {
   AdjCtx* x = h->adj;
+  x->forward_done = false;
+  // halo velocities of the recorded run for the recomputed steps (the
+  // ghost columns of u, v are constant over a run: they are the ones the
+  // state holds now)
+  hipError_t e = hipMemcpyAsync(x->macro[1], h->u, (size_t)h->plane_elems * sizeof(double),
+                                hipMemcpyDeviceToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(x->macro[2], h->v, (size_t)h->plane_elems * sizeof(double),
+                       hipMemcpyDeviceToDevice, h->stream);
   // east column of the initial state (T(Nx, :) as the first BC_g reads it)
-  hipError_t e = launch_teast_prologue(h);
+  if (e == hipSuccess) e = launch_teast_prologue(h);
   if (e != hipSuccess) return e;
   int te = 0;  // teast[te] holds the current column
   for (int n = 0; n < x->nsteps; ++n) {

@@ -8,12 +8,14 @@
 #include <sys/resource.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "bgk_internal.h"
 
@@ -173,6 +175,7 @@ void set_constants(bgk_handle* h) {
   c.noise_amp = h->ext.noise_amp * (p.t0 - p.t_in_sol);
   c.noise_seed = h->ext.noise_seed;
   c.crit_shift = h->ext.t_crit > 0.0 ? 273.0 - h->ext.t_crit : 0.0;
+  c.t_crit = h->ext.t_crit > 0.0 ? h->ext.t_crit : 273.0;
   c.t_ref = h->ext.t_ref;
   const double kvisc = c.cs2 * (p.tau_f - 0.5);
   c.u0 = p.re * kvisc / p.length;
@@ -275,6 +278,18 @@ int copy_small_back(bgk_handle* h, double* out, int n) {
                          h->stream),
           "hipMemcpyAsync(D2H)");
   HIP_TRY(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+  return BGK_OK;
+}
+
+// BGK_E_NONFINITE when one of out[0..n) is NaN or infinite: a sum over the
+// interior is non-finite exactly when a summed field holds a NaN or an
+// infinity (or the sum overflows), so the host test of the n results is
+// the detection.
+int nonfinite_check(const double* out, int n, const char* what) {
+  for (int k = 0; k < n; ++k)
+    if (!std::isfinite(out[k]))
+      return fail(BGK_E_NONFINITE, "%s: result %d is %g: the fields hold a NaN or an infinity",
+                  what, k, out[k]);
   return BGK_OK;
 }
 
@@ -401,6 +416,7 @@ int bgk_set_stream(bgk_handle* h, void* s) {
 int bgk_initialization(bgk_handle* h) {
   int rc = check_handle(h);
   if (rc) return rc;
+  bgk::adj_invalidate(h);
   if (h->prm.precision == BGK_FP32_DEV) return launched(bgk::launch_initialization32(h), "initialization");
   if (h->prm.precision == BGK_FP32_MOM || h->prm.precision == BGK_FP64_MOM)
     return launched(bgk::launch_initialization_mom(h), "initialization");
@@ -419,17 +435,9 @@ int bgk_set_extensions(bgk_handle* h, const bgk_ext* e) {
                 "need t_hot >= 0, k_ratio > 0, east_cold 0/1 with t_cold > 0, noise_amp >= 0, "
                 "t_crit >= 0, t_ref >= 0",
                 e->t_hot, e->k_ratio, e->east_cold, e->t_cold, e->noise_amp, e->t_crit, e->t_ref);
-  // every handle carries the conductivity ratio (one relaxation rate per
-  // node in the collision); the others need a moment handle
-  const bool mom = h->prm.precision == BGK_FP32_MOM || h->prm.precision == BGK_FP64_MOM;
-  const bool off = (e->t_hot == 0.0 || e->t_hot == h->prm.t0) && !e->east_cold &&
-                   e->noise_amp == 0.0 && (e->t_crit == 0.0 || e->t_crit == 273.0) &&
-                   e->t_ref == 0.0;
-  if (!mom && !off)
-    return fail(BGK_E_UNSUPPORTED, "physics extensions other than k_ratio need a moment handle "
-                                   "(precision BGK_FP32_MOM or BGK_FP64_MOM)");
   h->ext = *e;
   set_constants(h);
+  bgk::adj_invalidate(h);
   return BGK_OK;
 }
 
@@ -442,6 +450,7 @@ int bgk_upload(bgk_handle* h, int32_t field, const double* src, int64_t ld) {
     return fail(BGK_E_UNSUPPORTED, "a moment handle's state is set by bgk_initialization only");
   FieldRef r;
   if ((rc = resolve(h, field, &r))) return rc;
+  bgk::adj_invalidate(h);
   const bool direct = r.d64 && r.stride == 1;
   double* dst = direct ? r.d64 : h->tmp;
   HIP_TRY(hipMemcpy2DAsync(dst, (size_t)h->pitch * sizeof(double), src, (size_t)ld * sizeof(double),
@@ -491,6 +500,7 @@ int bgk_step(bgk_handle* h, int32_t nsteps) {
   if (rc) return rc;
   if (nsteps < 0) return fail(BGK_E_INVALID, "nsteps=%d", nsteps);
   if (nsteps == 0) return BGK_OK;
+  bgk::adj_invalidate(h);
   if (h->fused_ok) {
     if ((rc = launched(bgk::launch_teast_prologue(h), "teast prologue"))) return rc;
     const int prec = h->prm.precision;
@@ -534,7 +544,7 @@ int bgk_field_check(bgk_handle* h, double out[3]) {
   out[0] = s[0] / nn;
   out[1] = s[1] / nn;
   out[2] = s[2] / nn;
-  return BGK_OK;
+  return nonfinite_check(out, 3, "field_check");
 }
 
 int bgk_mass_check(bgk_handle* h, double out[3]) {
@@ -548,7 +558,7 @@ int bgk_mass_check(bgk_handle* h, double out[3]) {
   out[0] = (nn * h->prm.rho0 - s[0]) * 100.0 / (nn * h->prm.rho0);
   out[1] = s[0] / nn;
   out[2] = s[1] / nn;
-  return BGK_OK;
+  return nonfinite_check(out, 3, "mass_check");
 }
 
 int bgk_diag_sums(bgk_handle* h, double out[4]) {
@@ -558,7 +568,8 @@ int bgk_diag_sums(bgk_handle* h, double out[4]) {
   if (h->nx < 2)
     return fail(BGK_E_UNSUPPORTED, "diag reads T(1..3,j): needs Nx >= 2 (Nx=%d)", h->nx);
   if ((rc = launched(bgk::launch_diag_sums(h, h->small), "diag sums"))) return rc;
-  return copy_small_back(h, out, 4);
+  if ((rc = copy_small_back(h, out, 4))) return rc;
+  return nonfinite_check(out, 2, "diag sums");
 }
 
 int bgk_gather(bgk_handle* h, int32_t field, int32_t n, const int32_t* i, const int32_t* j,
@@ -621,6 +632,15 @@ int bgk_adj_setup_levels(bgk_handle* h, int32_t nsteps, int32_t seg_outer, int32
   for (int32_t p = 0; p < n_probes; ++p)
     if (pi[p] < 1 || pi[p] > h->nx || pj[p] < 1 || pj[p] > h->ny)
       return fail(BGK_E_INVALID, "probe %d at (%d,%d) is not an interior node", p, pi[p], pj[p]);
+  {
+    // each node carries one probe slot on the device: a repeated node would
+    // leave one of the two records unwritten
+    std::vector<int64_t> q((size_t)n_probes);
+    for (int32_t p = 0; p < n_probes; ++p) q[(size_t)p] = (int64_t)pj[p] * (h->nx + 2) + pi[p];
+    std::sort(q.begin(), q.end());
+    if (std::adjacent_find(q.begin(), q.end()) != q.end())
+      return fail(BGK_E_INVALID, "two probes share one node; give each node at most once");
+  }
   hipError_t e = bgk::adj_setup(h, nsteps, seg_outer, seg_inner, n_probes, pi, pj);
   if (e == hipErrorOutOfMemory)
     return fail(BGK_E_NOMEM, "adjoint checkpoints of %d steps (segments %d, %d) do not fit",
@@ -658,6 +678,9 @@ int bgk_adj_backward(bgk_handle* h, const double* seeds, double grad[3]) {
   if (rc) return rc;
   if (!h->adj) return fail(BGK_E_INVALID, "bgk_adj_setup first");
   if (!seeds || !grad) return fail(BGK_E_INVALID, "null seeds or gradient");
+  if (!bgk::adj_forward_done(h))
+    return fail(BGK_E_INVALID, "bgk_adj_forward first: no recorded forward run matches the "
+                               "current state and constants");
   hipError_t e = bgk::adj_backward(h, seeds, grad);
   if (e != hipSuccess) return hip_fail(e, "adjoint backward");
   return BGK_OK;

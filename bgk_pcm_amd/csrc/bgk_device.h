@@ -115,6 +115,7 @@ struct Consts {
   double noise_amp;    // initial T noise amplitude in kelvin (0: none)
   uint64_t noise_seed;
   double crit_shift;   // 273 - t_crit of the phase field (0: STEP_7)
+  double t_crit;       // critical temperature of the phase field (273: STEP_7)
   double t_ref;        // buoyancy reference temperature (0: STEP_7's absolute T)
 };
 
@@ -140,6 +141,12 @@ __host__ __device__ inline double node_uniform(uint64_t seed, int64_t q) {
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;
   return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Buoyancy of forces.f90:20, rho*grav*beta*T*(1 + phi/2), with T - t_ref
+// in place of T (t_ref = 0 subtracts an exact zero: STEP_7's bits).
+__device__ __forceinline__ double buoyancy(const Consts& c, double rho, double T, double phi) {
+  return rho * c.grav * c.beta * (T - c.t_ref) * (1.0 + phi / 2.0);
 }
 
 // Temperature of the east ghost column for row j (BC_g.f90:21-31: the
@@ -283,6 +290,9 @@ __device__ __forceinline__ NodeInit initial_node(const Consts& c, const Geom& gm
   s.u = c.u0;
   s.v = c.v0;
   if (i == 0) s.T = c.tin;
+  // extension: seeded noise on the interior temperature (0: STEP_7)
+  if (c.noise_amp > 0.0 && i >= 1 && i <= gm.nx && j >= 1 && j <= gm.ny)
+    s.T += c.noise_amp * (2.0 * node_uniform(c.noise_seed, (int64_t)j * (gm.nx + 2) + i) - 1.0);
   if (j == 0 || j == gm.ny + 1) {
     s.u = 0.0;
     s.v = 0.0;
@@ -331,9 +341,10 @@ __device__ __forceinline__ double phase_update(const Consts& c, double T, const 
   // front.  (With a non-finite T the reference would give NaN here too; T
   // itself carries the NaN either way.)
   if ((kp == 0.0 || 1.0 - pc == 0.0) && (km == 0.0 || 1.0 + pc == 0.0)) return pc;
-  // T_act = 0 and T_crit = 273 (phase_field.f90:15-16): both reaction rates
-  // take tanh of the same argument.
-  const double th = tanh_fdlibm(((T - 273.0) - 0.0) / c.t_width);
+  // T_act = 0 and T_crit = 273 (phase_field.f90:15-16; c.t_crit is 273
+  // unless the extension moves it): both reaction rates take tanh of the
+  // same argument.
+  const double th = tanh_fdlibm(((T - c.t_crit) - 0.0) / c.t_width);
   const double react_plus = 0.5 * (1.0 + th);
   const double react_minus = 0.5 * (1.0 - th);
   kp = (kp / 8.0) * react_plus;

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pairs64 A, Pairs64 B,
         gp[k] = equilibrium_k(c, k, c.tin, m.u[h], m.v[h]);
       } else {
         const int64_t h = (int64_t)sjc * P + nx + 1;
-        gp[k] = equilibrium_k(c, k, te_in[sjc], m.u[h], m.v[h]);
+        gp[k] = equilibrium_k(c, k, east_T(c, te_in, sjc), m.u[h], m.v[h]);
       }
     }
   }
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pairs64 A, Pairs64 B,
   // ---- forces (STEP_7/forces.f90:18-26)
   T = T - (c.dt_st * (phi - pc) / 2.0);
   {
-    double fy = rho * c.grav * c.beta * T * (1.0 + phi / 2.0);
+    double fy = buoyancy(c, rho, T, phi);
     const double fx = -(u * (1.0 - phi) * 0.5);
     fy = fy - v * (1.0 - phi) * 0.5;
     u = u + fx / (c.omega_f * rho);
@@ -177,17 +177,17 @@ __global__ __launch_bounds__(TILE_W) void k_fused64(Pairs64 A, Pairs64 B,
       }
       if (e_edge) {
         const int64_t h = r0 + nx + 1;
-        m.T[h] = te_in[j];
+        m.T[h] = east_T(c, te_in, j);
         pb[h] = pa[h];
         te_out[j] = Tav;
         if (s_edge) {
-          m.T[nx + 1] = te_in[1];
+          m.T[nx + 1] = east_T(c, te_in, 1);
           pb[nx + 1] = pa[P + nx + 1];
           B.p[6][nx + 1].x = A.p[6][nx + 1].x;
         }
         if (n_edge) {
           const int64_t hn = (int64_t)(ny + 1) * P + nx + 1;
-          m.T[hn] = te_in[ny];
+          m.T[hn] = east_T(c, te_in, ny);
           pb[hn] = pa[(int64_t)ny * P + nx + 1];
           B.p[7][hn].x = A.p[7][hn].x;
         }

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
         dg[k] = ghost_dev(c, k, c.tin, c.g_ref, m.u[h], m.v[h]);
       } else {
         const int64_t h = (int64_t)sjc * P + nx + 1;
-        dg[k] = ghost_dev(c, k, te_in[sjc], c.g_ref, m.u[h], m.v[h]);
+        dg[k] = ghost_dev(c, k, east_T(c, te_in, sjc), c.g_ref, m.u[h], m.v[h]);
       }
     }
   }
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
   dT = dT - (c.dt_st * (phi - pc) / 2.0);
   const double T = c.g_ref + dT;
   {
-    double fy = rho * c.grav * c.beta * T * (1.0 + phi / 2.0);
+    double fy = buoyancy(c, rho, T, phi);
     const double fx = -(u * (1.0 - phi) * 0.5);
     fy = fy - v * (1.0 - phi) * 0.5;
     u = u + fx / (c.omega_f * rho);
@@ -240,17 +240,17 @@ __global__ __launch_bounds__(TILE_W) void k_fused32(Pairs32 A, Pairs32 B,
       }
       if (e_edge) {
         const int64_t h = r0 + nx + 1;
-        m.T[h] = te_in[j];
+        m.T[h] = east_T(c, te_in, j);
         pb[h] = pa[h];
         te_out[j] = Tav;
         if (s_edge) {
-          m.T[nx + 1] = te_in[1];
+          m.T[nx + 1] = east_T(c, te_in, 1);
           pb[nx + 1] = pa[P + nx + 1];
           B.p[6][nx + 1].x = A.p[6][nx + 1].x;
         }
         if (n_edge) {
           const int64_t hn = (int64_t)(ny + 1) * P + nx + 1;
-          m.T[hn] = te_in[ny];
+          m.T[hn] = east_T(c, te_in, ny);
           pb[hn] = pa[(int64_t)ny * P + nx + 1];
           B.p[7][hn].x = A.p[7][hn].x;
         }

@@ -359,10 +359,7 @@ __global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* ph
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > gm.nx + 1 || j > gm.ny + 1) return;
   const int64_t q = (int64_t)j * gm.pitch + i;
-  NodeInit s0 = initial_node(c, gm, i, j, interf, west_bc);
-  // extension: seeded noise on the interior temperature
-  if (c.noise_amp > 0.0 && i >= 1 && i <= gm.nx && j >= 1 && j <= gm.ny)
-    s0.T += c.noise_amp * (2.0 * node_uniform(c.noise_seed, (int64_t)j * (gm.nx + 2) + i) - 1.0);
+  const NodeInit s0 = initial_node(c, gm, i, j, interf, west_bc);
   st.a[q] = make_double2(s0.rho, s0.T - kTcrit);
   static_cast<typename Vec<R>::V4*>(st.b)[q] = make4<R>((R)s0.u, (R)s0.v, R(0), R(0));
   static_cast<typename CType<R, EQF>::T*>(st.c)[q] = typename CType<R, EQF>::T{};

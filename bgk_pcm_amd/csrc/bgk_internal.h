@@ -123,6 +123,10 @@ void adj_info(const bgk_handle* h, int64_t* bytes, int32_t* n_states);
 hipError_t adj_forward(bgk_handle* h, double* probe_T_host);
 hipError_t adj_backward(bgk_handle* h, const double* seeds_host, double grad[3]);
 void adj_free(bgk_handle* h);
+// true once adj_forward has recorded the checkpoints the backward pass
+// replays; calls that change the state or the constants clear it
+bool adj_forward_done(const bgk_handle* h);
+void adj_invalidate(bgk_handle* h);
 
 }  // namespace bgk
 

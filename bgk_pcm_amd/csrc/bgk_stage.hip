@@ -213,7 +213,9 @@ __global__ void k_bc_g_we(Pops64 G, Macro m, Consts c, Geom gm) {
     m.T[at(0, j)] = c.tin;
   }
   {
-    const double ue = m.u[at(nx + 1, j)], ve = m.v[at(nx + 1, j)], te = m.T[at(nx, j)];
+    // the averaged T(Nx, j), or the cold-wall extension's fixed temperature
+    const double ue = m.u[at(nx + 1, j)], ve = m.v[at(nx + 1, j)];
+    const double te = c.east_cold ? c.t_cold : m.T[at(nx, j)];
     const double uu = ue * ue, vv = ve * ve, uv = 1.5 * (uu + vv), upv = ue + ve, umv = ue - ve;
     g[3][at(nx + 1, j)] = c.w[3] * te * (1.0 - 3.0 * ue + 4.5 * uu - uv);
     g[6][at(nx + 1, j)] = c.w[6] * te * (1.0 - 3.0 * umv + 4.5 * umv * umv - uv);
@@ -309,7 +311,7 @@ __global__ void k_forces(Macro m, const double* phi, const double* pp, Consts c,
   const double ph = phi[q];
   const double rho = m.rho[q];
   const double t = m.T[q] - (c.dt_st * (ph - pp[q]) / 2.0);
-  double fy = rho * c.grav * c.beta * t * (1.0 + ph / 2.0);
+  double fy = buoyancy(c, rho, t, ph);
   const double u = m.u[q], v = m.v[q];
   const double fx = -(u * (1.0 - ph) * 0.5);
   fy = fy - v * (1.0 - ph) * 0.5;

@@ -5,7 +5,7 @@
 // and a degree-5 rational correction in r^2/2).  The correction polynomial
 // is evaluated in the split form R1 + h2*R2 + h4*R3 that GNU libc 2.35 uses,
 // so the result equals glibc's tanh bit for bit (the reference calls glibc
-// tanh at STEP_7/phase_field.f90:128-129; tests/test_tanh.py checks the
+// tanh at STEP_7/phase_field.f90:24-25; tests/test_tanh.py checks the
 // equality on the host).  Every operation is a correctly rounded IEEE add,
 // multiply or divide, so the value is the same on gfx950 provided nothing is
 // contracted into FMA; the functions switch contraction off themselves.

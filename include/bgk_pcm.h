@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define BGK_ABI_VERSION 3
+#define BGK_ABI_VERSION 4
 
 /* return codes */
 enum {
@@ -41,7 +41,9 @@ enum {
   BGK_E_HIP = -2,         /* a HIP runtime call failed                         */
   BGK_E_NOMEM = -3,       /* device allocation failed                          */
   BGK_E_UNSUPPORTED = -4, /* valid request this handle cannot serve            */
-  BGK_E_NODEVICE = -5     /* no GPU visible / wrong device ordinal             */
+  BGK_E_NODEVICE = -5,    /* no GPU visible / wrong device ordinal             */
+  BGK_E_NONFINITE = -6    /* a reduction met a NaN or an infinity in the fields
+                             it sums; its outputs are written (non-finite)     */
 };
 
 /* storage precision of the populations (bgk_params.precision) */
@@ -219,18 +221,21 @@ int bgk_moments(bgk_handle* h);     /* STEP_7/moments.f90:8-27     */
 int bgk_step(bgk_handle* h, int32_t nsteps);
 
 /* STEP_7/field_check.f90:11-24: out = { <rho>, <T>, <phi> } over the
- * interior. */
+ * interior.  Returns BGK_E_NONFINITE (out still written) when a mean is not
+ * finite: the reference prints NaN and runs on; a caller that wants that
+ * behaviour ignores this code. */
 int bgk_field_check(bgk_handle* h, double out[3]);
 
 /* STEP_7/mass_check.f90:8-32: out = { mass (percent defect),
- * <rho>, <T> } over the interior. */
+ * <rho>, <T> } over the interior; BGK_E_NONFINITE as bgk_field_check. */
 int bgk_mass_check(bgk_handle* h, double out[3]);
 
 /* Field sums used by STEP_7/diag.f90:63-104:
  * out[0] = sum_j Ny*((Tin - T(2,j))/2)/Delta_T            (before /Ny)
  * out[1] = sum_j Ny*|(-3T(1,j)+4T(2,j)-T(3,j))/2|/Delta_T (before /Ny)
  * out[2] = number of interior nodes with phi > 0
- * out[3] = sum of i over interior nodes with phi == 0 */
+ * out[3] = sum of i over interior nodes with phi == 0
+ * BGK_E_NONFINITE (out still written) when out[0] or out[1] is not finite. */
 int bgk_diag_sums(bgk_handle* h, double out[4]);
 
 /* Values of one field at n nodes (i[k], j[k]) (0-based halo-inclusive

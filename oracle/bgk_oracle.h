@@ -80,7 +80,7 @@ void oracle_diag_sums(const oracle_t* s, double out[4]);
  * only defined when mod(it,100)==0; `valid` flags which fronts were set. */
 void oracle_diag(const oracle_t* s, int it, double out[12], int* valid);
 
-/* glibc tanh, as called by the reference (STEP_7/phase_field.f90:128-129) */
+/* glibc tanh, as called by the reference (STEP_7/phase_field.f90:24-25) */
 double oracle_tanh(double x);
 
 #ifdef __cplusplus

@@ -26,6 +26,27 @@ binary VTK files of STEP_CUF as bytes (bin_<name>, uint8) and STEP_7's
 dumpFile.dat as text (text_dumpFile), with `params` and `nsteps`
 (`python tests/golden/make_golden.py formats` writes only this one).
 
+long_runs.json holds sha256 digests of the final rho, u, v, T and phi (full
+(Ny+2) x (Nx+2) float64 arrays, little-endian, row j after row j-1, halo
+included) of the benchmark configurations at their full lengths, where a
+stored array would be too large (`python tests/golden/make_golden.py long`,
+about two and a half hours on 8 cores):
+  cavity2048_ra1e6_pr002   the 2048^2 FP64 config (Ra 1e6, Stefan 0.1,
+                           tau_f 0.51 / tau_g 1.0 for Pr 0.02) as STEP_7 can
+                           express it (no noise, its averaged-T east wall):
+                           written by the reference itself,
+                           oracle/_ref/openmp_h (STEP_OPENMP, whose per-node
+                           arithmetic is STEP_7's: the same bits)
+  cavity2048_literal       the same with the config's cold east wall at
+                           T_in_sol and seeded initial noise: written by the
+                           C oracle (the extensions have no reference
+                           counterpart)
+  cavity256_literal        the 256^2 config (tau_f 0.8, tau_g 15.5, grav
+                           1e-4, cold east wall, noise) after 10000 steps:
+                           the C oracle
+Each entry also keeps min, max and mean of every field, for a readable
+report when a digest differs.
+
 driver_cav24x20.npz instead holds every file the complete reference program
 (oracle/_ref/step7/bgk_thermal, its own main.f90) writes for a 24x20,
 2000-step input, as text: file_<name> per file, the input.dat records in
@@ -121,6 +142,65 @@ def formats_fixture():
     return path
 
 
+LONG = HERE / "long_runs.json"
+
+
+def field_digests(fields: dict) -> dict:
+    """sha256 of each field's full array (float64, little-endian, C order of
+    the (Ny+2, Nx+2) array) and its min / max / mean."""
+    out = {}
+    for f in ("rho", "u", "v", "T", "phi"):
+        a = np.ascontiguousarray(fields[f], dtype="<f8")
+        out[f] = {"sha256": hashlib.sha256(a.tobytes()).hexdigest(),
+                  "min": float(a.min()), "max": float(a.max()), "mean": float(a.mean())}
+    return out
+
+
+def long_cases():
+    """(name, params, nsteps, oracle extensions or None for the reference)."""
+    from bgk_pcm_amd.params import grav_for_rayleigh
+    p2 = Params(nx=2048, ny=2048, length=2048.0, tau_f=0.51, tau_g=1.0, stefan_number=0.1,
+                iter=20000)
+    p2 = p2.replace(grav=grav_for_rayleigh(p2, 1e6))
+    p1 = Params(nx=256, ny=256, length=256.0, tau_f=0.8, tau_g=15.5, grav=1e-4,
+                stefan_number=0.1, iter=10000)
+    literal = dict(t_cold=273.0, noise_amp=1e-3, noise_seed=2024)
+    return [("cavity256_literal", p1, 10000, literal),
+            ("cavity2048_ra1e6_pr002", p2, 20000, None),
+            ("cavity2048_literal", p2, 20000, literal)]
+
+
+def long_fixture(only=None) -> None:
+    import time
+    from oracle_py import Oracle
+    from oracle_py import read_snapshot
+    data = json.loads(LONG.read_text()) if LONG.exists() else {}
+    for name, p, nsteps, ext in long_cases():
+        if only and name not in only:
+            continue
+        t0 = time.time()
+        if ext is None:
+            exe = REF_DIR / "openmp_h" / "ref_harness"
+            with tempfile.TemporaryDirectory(prefix="bgklong_", dir="/tmp") as wd:
+                write_input_dat(p, os.path.join(wd, "input.dat"))
+                os.mkdir(os.path.join(wd, "snap"))
+                subprocess.run([str(exe), str(nsteps), "1", "0", os.path.join(wd, "snap"), "0"],
+                               cwd=wd, check=True, capture_output=True)
+                snap = read_snapshot(os.path.join(wd, "snap", "final.bin"), p.nx, p.ny)
+                fields = {f: snap[FIELD_NAMES.index(f)] for f in ("rho", "u", "v", "T", "phi")}
+            source = "reference STEP_OPENMP (oracle/_ref/openmp_h/ref_harness)"
+        else:
+            o = Oracle(p, **ext)
+            o.step(nsteps)
+            fields = {f: o.field(f).copy() for f in ("rho", "u", "v", "T", "phi")}
+            o.close()
+            source = "C oracle (oracle/liboracle.so) with extensions " + json.dumps(ext)
+        data[name] = {"params": asdict(p), "nsteps": nsteps, "extensions": ext,
+                      "source": source, "digests": field_digests(fields)}
+        LONG.write_text(json.dumps(data, indent=1) + "\n")
+        print(f"{name}: {time.time() - t0:.0f} s", flush=True)
+
+
 def update_manifest(path) -> None:
     mf = HERE / "MANIFEST.sha256"
     lines = [l for l in mf.read_text().splitlines() if not l.endswith("  " + path.name)]
@@ -131,6 +211,9 @@ def update_manifest(path) -> None:
 def main() -> None:
     if len(sys.argv) > 1 and sys.argv[1] == "formats":
         update_manifest(formats_fixture())
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "long":
+        long_fixture(sys.argv[2:] or None)
         return
     manifest = []
     for name, (p, nsteps, window, stages, fields, diag_every) in CASES.items():

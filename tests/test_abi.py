@@ -36,7 +36,7 @@ def test_library_exports_every_declared_entry_point():
 
 
 def test_abi_version():
-    assert lattice.load_library().bgk_abi_version() == 3
+    assert lattice.load_library().bgk_abi_version() == 4
 
 
 @pytest.mark.parametrize("cname,ctype", [("bgk_params", BgkParamsC),

@@ -31,6 +31,22 @@ EXE = ROOT / "bgk_pcm_amd" / "bin" / "bgk_thermal_gpu"
 FIXTURE = ROOT / "tests" / "golden" / "driver_cav24x20.npz"
 
 
+def require_fresh_driver():
+    """The driver binary is built (not committed) by __graft_entry__.build();
+    `make -q` exits non-zero when it is missing or older than its sources or
+    the library, which would test a stale program."""
+    r = subprocess.run(["make", "-q", "-C", str(ROOT / "bgk_pcm_amd" / "fortran")],
+                       capture_output=True, text=True)
+    assert EXE.exists() and r.returncode == 0, (
+        f"{EXE} is missing or older than bgk_pcm_amd/fortran/*.f90 or libbgk_pcm.so: "
+        "rebuild with __graft_entry__.build()")
+
+
+@pytest.fixture(autouse=True)
+def fresh_driver():
+    require_fresh_driver()
+
+
 @pytest.mark.parametrize("mode", [[], ["--stages"]], ids=["fused", "stages"])
 def test_driver_writes_the_reference_files(tmp_path, mode):
     d = np.load(FIXTURE, allow_pickle=False)
@@ -112,26 +128,27 @@ def test_driver_restart_continues_the_run(tmp_path):
 
 def test_driver_extensions_namelist(tmp_path):
     """--ext FILE reads the namelist &bgk_ext: with the conductivity ratio
-    1.7 on the FP64 path the dump after 300 steps equals the oracle run
-    with the same ratio bit for bit; an extension the handle refuses (a
-    hot-wall temperature needs moment storage) stops the run with the
-    library's message."""
+    1.7, a hot wall at 284 K and the cold east wall at 273.5 K on the FP64
+    path the dump after 300 steps equals the oracle run with the same
+    extensions bit for bit; an extension the library refuses (a negative
+    conductivity ratio) stops the run with the library's message."""
     from bgk_pcm_amd import vtk
     from oracle_py import Oracle
     p = Params(nx=24, ny=20, length=24.0, grav=1e-3, iter=300, noutput=1)
     write_input_dat(p, tmp_path / "input.dat")
-    (tmp_path / "ext.nml").write_text("&bgk_ext\n  k_ratio = 1.7\n/\n")
-    print("driver --ext (k_ratio 1.7), 300 steps", flush=True)
+    (tmp_path / "ext.nml").write_text(
+        "&bgk_ext\n  k_ratio = 1.7\n  t_hot = 284.0\n  t_cold = 273.5\n/\n")
+    print("driver --ext (k_ratio 1.7, t_hot 284, t_cold 273.5), 300 steps", flush=True)
     r = subprocess.run([str(EXE), "--ext", "ext.nml", "--dump-out"], cwd=tmp_path,
                        capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr[-2000:]
     d = vtk.read_dump(tmp_path / "dumpFile.dat", p.nx, p.ny)
-    o = Oracle(p, k_ratio=1.7)
+    o = Oracle(p, k_ratio=1.7, t_hot=284.0, t_cold=273.5)
     o.step(300)
     for f in ("rho", "u", "v", "T", "phi"):
         assert np.array_equal(d[f][1:-1, 1:-1], o.field(f)[1:-1, 1:-1]), f
-    (tmp_path / "hot.nml").write_text("&bgk_ext t_hot = 284.0 /\n")
-    r = subprocess.run([str(EXE), "--ext", "hot.nml"], cwd=tmp_path, capture_output=True,
+    (tmp_path / "bad.nml").write_text("&bgk_ext k_ratio = -1.0 /\n")
+    r = subprocess.run([str(EXE), "--ext", "bad.nml"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=240)
     assert r.returncode != 0
-    assert "need a moment handle" in r.stderr, r.stderr[-500:]
+    assert "k_ratio > 0" in r.stderr, r.stderr[-500:]

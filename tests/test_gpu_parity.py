@@ -171,3 +171,45 @@ def test_upload_download_roundtrip_all_fields():
             a = rng.standard_normal((p.ny + 2, p.nx + 2))
             lat.upload(f, a)
             assert np.array_equal(lat.download(f), a), f
+
+
+EXT_ALL = dict(t_hot=284.5, k_ratio=1.6, t_cold=273.5, noise_amp=2e-3, noise_seed=77,
+               t_crit=274.0, t_ref=278.0)
+
+
+@pytest.mark.parametrize("rounding", [0, 1])
+@pytest.mark.parametrize("ext_name", ["t_hot", "cold_noise", "t_crit", "t_ref", "all"])
+def test_extensions_are_bit_exact_on_both_fp64_paths(rounding, ext_name):
+    """Every physics extension on an FP64 population handle (hot-wall
+    temperature, cold east wall, seeded initial noise, T_crit of the phase
+    field, Boussinesq reference of the buoyancy, with the conductivity ratio
+    in "all"): the fused kernel over 200 steps and the per-subroutine
+    kernels over 30 steps against the oracle with the same extensions, bit
+    for bit, and each extension changes the result."""
+    ext = {"t_hot": dict(t_hot=284.5), "cold_noise": dict(t_cold=273.5, noise_amp=2e-3,
+                                                          noise_seed=77),
+           "t_crit": dict(t_crit=274.0), "t_ref": dict(t_ref=278.0), "all": EXT_ALL}[ext_name]
+    p = Params(nx=67, ny=45, rounding=rounding, re=3.0, length=45.0, grav=1e-3)
+    print(f"extensions {ext}, 67x45, rounding={rounding}: fused 200 steps", flush=True)
+    o = Oracle(p, **ext)
+    o.step(200)
+    want = {f: o.field(f).copy() for f in STATE + POPS}
+    with Lattice(p, **ext) as lat:
+        lat.step(200)
+        got = gpu_state(lat, STATE + POPS)
+    assert compare_state(got, want, STATE) == []
+    assert compare_interior(got, want, POPS) == []
+    o1 = Oracle(p)
+    o1.step(200)
+    assert not np.array_equal(o1.field("T"), want["T"])
+    print(f"extensions {ext_name}: per-subroutine 30 steps", flush=True)
+    o = Oracle(p, **ext)
+    o.step(30)
+    with Lattice(p, **ext) as lat:
+        for _ in range(30):
+            for s in Lattice.STAGES:
+                lat.stage(s)
+        got = gpu_state(lat, STATE + POPS)
+    want = {f: o.field(f) for f in STATE + POPS}
+    assert compare_state(got, want, STATE) == []
+    assert compare_interior(got, want, POPS) == []

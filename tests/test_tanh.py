@@ -1,7 +1,7 @@
 """The phase-field tanh of the HIP kernels (bgk_pcm_amd/csrc/bgk_tanh.h),
 compiled for the host, equals glibc's tanh bit for bit.
 
-The reference calls glibc tanh (STEP_7/phase_field.f90:128-129, amdflang
+The reference calls glibc tanh (STEP_7/phase_field.f90:24-25, amdflang
 emits a libm call).  The header reimplements the published fdlibm method
 with glibc's evaluation of the expm1 correction polynomial; this test runs
 it on the host over the phase field's argument range ((T-273)/T_width for
