@@ -1,51 +1,55 @@
 """Throughput of the BGK_PCM time step on one MI355X.
 
-Workload: the 4096^2 FP32-storage config, a melting cavity with Ra = 1e7
-(grav set by the reference's definition), Stefan number 0.05 and a
-solid/liquid conductivity ratio of 2 (the k_ratio extension), otherwise the reference benchmark input UTIL/input.dat.0 (no-slip
-walls, tau_f=1, tau_g=0.51: the config's Pr 0.02 diverges at this Ra),
-advanced by the fused step with FP32 moment storage (per node rho and
-T - 273 in FP64; u, v and the thermal non-equilibrium moments in FP32; FP64
-arithmetic).  One "step" is
+Workload: the 4096^2 FP32-storage configuration (bgk_pcm_amd/configs.py,
+config3): a melting cavity at Ra = 1e7 and Pr = 0.02 (tau_f 0.51, tau_g 1.0;
+grav from the reference's definition), Stefan number 0.05 and a solid/liquid
+conductivity ratio of 2, otherwise the reference benchmark input
+UTIL/input.dat.0 (no-slip walls), advanced by the fused step with FP32
+moment storage (per node rho and T - 273 in FP64; u, v and the flow and
+thermal non-equilibrium moments in FP32; FP64 arithmetic).  One "step" is
 one time step of the whole lattice (BC_f ... moments of the reference time
 loop).  Inputs are resident in HBM before the timed region.
 
+Order of the run: the error runs first (20000 steps of each storage, which
+also bring the GPU to its steady clocks), then the timed steps.
+
 Printed JSON line (one):
-  value          MLUPS of the moment-storage step = Nx*Ny*steps/t/1e6
-  roofline       algorithmic HBM bytes per launch (88 B per lattice update
-                 with tau_f = 1: 36 B of moments read + written, phi FP64
-                 read + written; 112 B for tau_f != 1) over the fused
-                 kernel's mean launch time, measured with HIP events on the
-                 stream the kernel runs on; vs 8 TB/s
-  fp32_population the fused step with FP32 population storage (f - w*rho0,
-                 g - w*T_crit; 160 B per update), same workload and
-                 measurements
+  value          MLUPS of the moment-storage step = Nx*Ny*K/t/1e6 over the K
+                 timed steps (one bgk_step call after W warm-up steps)
+  roofline       algorithmic HBM bytes per launch (112 B per lattice update:
+                 48 B of moments read + written, phi FP64 read + written)
+                 over the fused kernel's time per launch, measured with HIP
+                 events on the stream the kernel runs on as the difference of
+                 a 400- and a 200-step call (which cancels the last launch of
+                 a call, that also writes every macroscopic field, reported
+                 as last_launch_extra_ms), median of three; vs 8 TB/s
+  fp32_population the fused step with FP32 population storage (f -
+                 w*rho_ref(row), g - w*T_crit; 160 B per update), same
+                 workload and measurements
   fp64           the FP64 fused kernel (304 B per update), likewise
-  fp32_mom_error relative max-norm error (per field, liquid fraction for
-                 phi) of the FP32 moment run against the FP64 moment run
-                 with the same conductivity ratio after 1/4, 1/2 and all of
-                 --error-steps (default 20000, the config's run length)
-  fp32_error     the errors after --error-steps of both FP32 storages
-                 against the FP64 population run with clean FP64
-                 temporaries and against the STEP_7-rounding FP64 run, and
-                 of the STEP_7 run against the clean one
-  adjoint        gradient of a probe-temperature loss with respect to
-                 (stefan_number, k_ratio, t_hot) by the GPU adjoint on the
-                 adjoint workload (512^2, FP64 moments, 2000 steps,
-                 checkpoint every 100, 16 seeded probes, target trajectory
-                 at St 0.12, T_h 1.05, k_ratio 1.5), central differences
-                 beside it, and the cost of the backward pass (recompute +
-                 adjoint) per step over the forward step
+  fp32_error     relative max-norm error (per field, liquid fraction for
+                 phi) after 5000, 10000 and 20000 steps of the FP32 moment
+                 and FP32 population storages, of the FP64 moment storage and
+                 of the STEP_7-rounding FP64 run, all against the clean FP64
+                 population run from the same initial state; within_1e-5 per
+                 storage is the maximum over fields and marks against 1e-5
   slab           the moment step on one GPU's share of an 8-GPU slab
                  decomposition of 8192^2: 8192 x 1026 nodes (1024 rows and
                  the two rows a neighbour would send), same measurements
-  adjoint_large  the large adjoint workload (8192^2, FP32 moment storage,
-                 Ra 1e7, St 0.1, 5000 steps, two levels of checkpoints every
-                 289 and 17 steps, 16 seeded probes, the same kind of target
-                 trajectory): device bytes of the handle and of the
-                 adjoint's checkpoints, forward and backward time per step
-                 and their ratio; and the same gradient by the FP32 and the
-                 FP64 adjoint at 1024^2 with their relative difference
+  adjoint        the adjoint configuration (config4: 512^2, FP64 moments,
+                 2000 steps, checkpoint every 100, 16 seeded probes, target
+                 trajectory at St 0.12, T_h 1.05, k_ratio 1.5): gradient by
+                 the GPU adjoint, central differences beside it, and the cost
+                 of the backward pass (recompute + adjoint) per step over the
+                 forward step
+  adjoint_large  the large adjoint configuration (config5: 8192^2, FP32
+                 moments, Ra 1e7, St 0.1, 5000 steps, two levels of
+                 checkpoints every 289 and 17 steps, 16 seeded probes): device
+                 bytes of the handle and of the adjoint's checkpoints,
+                 forward and backward time per step and their ratio, the
+                 adjoint kernel's time per launch against the forward
+                 kernel's; and the same gradient by the FP32 and the FP64
+                 adjoint at 1024^2 with their relative difference
   cpu_baseline   the reference's own STEP_OPENMP time step (compiled from its
                  sources into oracle/_ref) timed on this host's cores on a
                  bounded sample of the same workload; the CPU oracle (a port)
@@ -55,7 +59,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
                        [--error-steps E] [--cpu-steps S] [--no-cpu-baseline]
                        [--no-fp64] [--no-adjoint] [--no-adjoint-large]
                        [--adjoint-large-steps N] [--adjoint-large-only]
-                       [--calibrate-copy MiB]
+                       [--headline-only] [--calibrate-copy MiB]
 """
 from __future__ import annotations
 
@@ -64,6 +68,7 @@ import json
 import os
 import pathlib
 import re
+import statistics
 import subprocess
 import sys
 import tempfile
@@ -79,6 +84,7 @@ METRIC = "MLUPS (D2Q9 f+g) on 1 MI355X: FP32-store vs FP64 at <=1e-5 error; % HB
 BYTES_PER_LUP = {"mom_eq": 36 * 2 + 8 * 2, "mom": 48 * 2 + 8 * 2,
                  "fp32": 18 * 4 * 2 + 8 * 2, "fp64": 18 * 8 * 2 + 8 * 2}
 FIELDS = ("rho", "u", "v", "T", "phi")
+BOUND = 1e-5
 
 
 def parse():
@@ -97,55 +103,83 @@ def parse():
     ap.add_argument("--adjoint-large-only", action="store_true",
                     help="only the large adjoint workload, without the 1024^2 check "
                          "(for profiling it)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the timed steps and the roofline of the three storages "
+                         "(for profiling them)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="after the timed region, run a device-to-device copy of this size "
                          "(reference kernel for HBM counter scaling in profiles)")
     return ap.parse_args()
 
 
-RAYLEIGH = 1e7
-STEFAN = 0.05  # the 4096^2 FP32-storage config
-K_RATIO = 2.0
+def progress(msg: str) -> None:
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
 
-def cavity(n: int):
-    """The bench workload: UTIL/input.dat.0's cavity on an n x n lattice with
-    grav set for Ra = 1e7 (the Rayleigh number of the 4096^2 FP32 config) by
-    the reference's own definition.  Pr = 50 stays that of the reference
-    relaxation times; grav = 2.5e-4 itself would put a hydrostatic density
-    drop of ~3 across 4096 rows (STEP_7's buoyancy uses the absolute T) and
-    the run diverges after a few thousand steps."""
-    from bgk_pcm_amd.params import Params, grav_for_rayleigh
-    p = Params(nx=n, ny=n, iter=20000, length=float(n))
-    return p.replace(grav=grav_for_rayleigh(p, RAYLEIGH))
+def workload(n: int):
+    """config3 on an n x n lattice (grav for Ra 1e7 at that size)."""
+    from bgk_pcm_amd import configs
+    w = configs.config3()
+    if n != w.params.nx:
+        w.params = configs.cavity(n, 1e7, stefan_number=w.params.stefan_number,
+                                  iter=w.params.iter, rounding=w.params.rounding,
+                                  **configs.PR002)
+    return w
 
 
-def timed(lat, stream, steps: int, warmup: int):
-    """(kernel ms per step from HIP events, wall ms per step)."""
+def events(stream):
     import torch
-    lat.step(warmup)
-    lat.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    lat.step(steps)
-    ev1.record(stream)
-    lat.synchronize()
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) * 1e3 / steps
-    return ev0.elapsed_time(ev1) / steps, wall
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
-def gpu_run(p, stream, steps, warmup, ext=None):
+def call_ms(lat, stream, n: int) -> float:
+    """HIP-event time of one bgk_step(n) call on the handle's stream."""
+    e0, e1 = events(stream)
+    e0.record(stream)
+    lat.step(n)
+    e1.record(stream)
+    lat.synchronize()
+    return e0.elapsed_time(e1)
+
+
+def per_launch(lat, stream, m: int = 200, reps: int = 3):
+    """(kernel ms per launch, extra ms of a call's last launch): a call of n
+    steps costs the prologue, n - 1 plain launches and a last launch that
+    also writes every macroscopic field, so T(2m) - T(m) = m plain launches;
+    medians over reps."""
+    k, extra = [], []
+    for _ in range(reps):
+        t1 = call_ms(lat, stream, m)
+        t2 = call_ms(lat, stream, 2 * m)
+        k.append((t2 - t1) / m)
+        extra.append(t1 - m * (t2 - t1) / m)
+    return statistics.median(k), statistics.median(extra)
+
+
+def measure(p, stream, steps, warmup, kind, kernel, ext=None):
+    """MLUPS over `steps` timed steps (wall clock, one bgk_step call after
+    `warmup` steps), and the roofline figures of the fused kernel from HIP
+    events on its stream."""
     from bgk_pcm_amd.lattice import Lattice
-    lat = Lattice(p, **(ext or {}))
-    lat.set_stream(stream.cuda_stream)
-    kms, wms = timed(lat, stream, steps, warmup)
-    fc = lat.field_check()
-    lat.close()
-    return kms, wms, fc
+    lups = p.nx * p.ny
+    with Lattice(p, **(ext or {})) as lat:
+        lat.set_stream(stream.cuda_stream)
+        lat.step(warmup)
+        lat.synchronize()
+        t0 = time.perf_counter()
+        lat.step(steps)
+        lat.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        kms, extra = per_launch(lat, stream)
+        lat.field_check()  # raises on a non-finite field
+    nbytes = BYTES_PER_LUP[kind] * lups
+    ach = nbytes / (kms * 1e-3) / 1e9
+    return {"mlups": round(lups / (wall * 1e-3) / 1e6, 1), "ms_per_step": round(wall, 5),
+            "kernel": kernel, "kernel_ms": round(kms, 5),
+            "last_launch_extra_ms": round(extra, 5),
+            "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+            "bytes_per_lup": BYTES_PER_LUP[kind], "algorithmic_bytes_per_launch": nbytes,
+            "traffic": traffic_from_profile(p.nx, kind)}
 
 
 def rel_errors(a, b):
@@ -161,50 +195,39 @@ def rel_errors(a, b):
     return out
 
 
-def fp32_error(p64, storages: dict, steps: int, ext=None):
-    """Errors after `steps` steps from the same initial state: each FP32
-    storage run against the FP64 run with clean FP64 temporaries and against
-    the STEP_7-rounding FP64 run, and the STEP_7 run against the clean one
-    (the reference's own FP32-temporary noise floor); ext: physics
-    extensions of every run."""
-    from bgk_pcm_amd.lattice import Lattice
-    from bgk_pcm_amd.params import FP64, ROUND_CLEAN, ROUND_STEP7
-    assert p64.precision == FP64
-    runs = {}
-    todo = [("fp64_step7", p64.replace(rounding=ROUND_STEP7)),
-            ("fp64_clean", p64.replace(rounding=ROUND_CLEAN))] + list(storages.items())
-    for name, p in todo:
-        with Lattice(p, **(ext or {})) as lat:
-            lat.step(steps)
-            runs[name] = {f: lat.download(f) for f in FIELDS}
-    out = {"steps": steps}
-    for name in storages:
-        out[f"{name}_vs_fp64_clean"] = rel_errors(runs[name], runs["fp64_clean"])
-        out[f"{name}_vs_fp64_step7"] = rel_errors(runs[name], runs["fp64_step7"])
-    out["fp64_step7_vs_fp64_clean"] = rel_errors(runs["fp64_step7"], runs["fp64_clean"])
-    return out
-
-
-def mom_error_vs_fp64_mom(p, ext: dict, error_steps: int):
-    """The FP32 moment run against the FP64 moment run with the same
-    physics extensions (the same model in FP64; the clean FP64 population
-    kernel agrees with it to 1e-10 at small sizes), both from the same
+def errors(w, error_steps: int):
+    """Every storage against the clean FP64 population run from the same
     initial state, after 1/4, 1/2 and all of error_steps."""
     from bgk_pcm_amd.lattice import Lattice
-    from bgk_pcm_amd.params import FP32_MOM, FP64_MOM
+    from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, FP64, FP64_MOM, ROUND_STEP7
     marks = sorted({max(error_steps // 4, 1), max(error_steps // 2, 1), error_steps})
-    errs = []
-    with Lattice(p.replace(precision=FP64_MOM), **ext) as l64, \
-            Lattice(p.replace(precision=FP32_MOM), **ext) as l32:
-        done = 0
-        for m in marks:
-            l64.step(m - done)
-            l32.step(m - done)
-            done = m
-            e = rel_errors({f: l32.download(f) for f in FIELDS},
-                           {f: l64.download(f) for f in FIELDS})
-            errs.append(dict(steps=m, **e))
-    return errs
+    p = w.params
+
+    def run(pp):
+        out = {}
+        with Lattice(pp, **w.ext) as lat:
+            done = 0
+            for m in marks:
+                lat.step(m - done)
+                done = m
+                out[m] = {f: lat.download(f) for f in FIELDS}
+        return out
+
+    progress(f"errors: clean FP64 populations, {error_steps} steps")
+    truth = run(p.replace(precision=FP64))
+    res = {"vs": "clean FP64 populations (same initial state)", "marks": marks}
+    for name, pp in (("fp32_moments", p.replace(precision=FP32_MOM)),
+                     ("fp32_populations", p.replace(precision=FP32_DEV)),
+                     ("fp64_moments", p.replace(precision=FP64_MOM)),
+                     ("fp64_step7_rounding", p.replace(precision=FP64, rounding=ROUND_STEP7))):
+        progress(f"errors: {name}")
+        got = run(pp)
+        res[name] = [dict(steps=m, **rel_errors(got[m], truth[m])) for m in marks]
+        del got
+    res["within_1e-5"] = {name: max(max(v for k, v in e.items() if k != "steps")
+                                    for e in res[name]) <= BOUND
+                          for name in ("fp32_moments", "fp32_populations", "fp64_moments")}
+    return res
 
 
 def cpu_model() -> str:
@@ -261,25 +284,24 @@ def traffic_from_profile(size: int, kind: str):
         return None
 
 
-def adjoint_run(n: int = 512, nsteps: int = 2000, seg: int = 100, n_probes: int = 16):
-    """The adjoint workload: gradient at the reference's parameters (St 0.1,
-    T_h = T0, k_ratio 1) of the mean squared error of T at 16 seeded probes
-    against the trajectory at St 0.12, T_h 1.05 (theta units, T0 - T_in_sol
-    = 10 K per unit above T_crit = 273), k_ratio 1.5; grav for Ra = 1e6 (the
-    2048^2 config's Rayleigh number)."""
+def adjoint_run():
+    """config4: gradient at (St, k_ratio, T_h) = (0.1, 1, 1) of the mean
+    squared error of T at 16 seeded probes against the trajectory at (0.12,
+    1.5, 1.05); central differences with steps 1e-6 in St, k_ratio and T_h
+    (theta units: 1e-5 K of t_hot)."""
     import numpy as np
+    from bgk_pcm_amd import configs
     from bgk_pcm_amd.adjoint import seeded_probes
     from bgk_pcm_amd.lattice import Lattice
-    from bgk_pcm_amd.params import FP64_MOM, ROUND_CLEAN, Params, grav_for_rayleigh
-    base = Params(nx=n, ny=n, length=float(n), rounding=ROUND_CLEAN, precision=FP64_MOM,
-                  stefan_number=0.1)
-    base = base.replace(grav=grav_for_rayleigh(base, 1e6))
+    w = configs.config4()
+    base, nsteps, seg = w.params, w.nsteps, 100
     dT = base.t0 - base.t_in_sol
-    pi, pj = seeded_probes(n, n, n_probes, seed=0)
+    pi, pj = seeded_probes(base.nx, base.ny, 16, seed=0)
 
     def forward(theta, keep=False):
         st, kr, th = theta
-        lat = Lattice(base.replace(stefan_number=st), initialize=False, t_hot=th, k_ratio=kr)
+        lat = Lattice(base.replace(stefan_number=st), initialize=False,
+                      t_hot=configs.t_hot_of(base, th), k_ratio=kr)
         lat.initialization()
         lat.adj_setup(nsteps, seg, pi, pj)
         t0 = time.perf_counter()
@@ -290,14 +312,15 @@ def adjoint_run(n: int = 512, nsteps: int = 2000, seg: int = 100, n_probes: int 
         lat.close()
         return rec
 
-    target = forward((0.12, 1.5, 273.0 + 1.05 * dT))
-    theta0 = np.array([0.1, 1.0, base.t0])
+    target = forward(configs.ADJ_TARGET)
+    theta0 = np.array(configs.ADJ_THETA0)
     lat, rec, t_fwd = forward(theta0, keep=True)
     seeds = 2.0 * (rec - target) / rec.size
     t0 = time.perf_counter()
     grad = lat.adj_backward(seeds)
     t_bwd = time.perf_counter() - t0
     lat.close()
+    grad[2] *= dT  # per theta unit, as the differences below
     # plain forward steps of the same handle type, for the cost ratio
     with Lattice(base) as lat:
         lat.step(50)
@@ -308,43 +331,40 @@ def adjoint_run(n: int = 512, nsteps: int = 2000, seg: int = 100, n_probes: int 
         t_plain = time.perf_counter() - t0
     loss = lambda r: float(((r - target) ** 2).mean())
     fd = []
-    for c, h in enumerate((1e-6, 1e-6, 1e-6 * dT)):
+    for c in range(3):
         tp, tm = theta0.copy(), theta0.copy()
-        tp[c] += h
-        tm[c] -= h
-        fd.append((loss(forward(tp)) - loss(forward(tm))) / (2 * h))
+        tp[c] += 1e-6
+        tm[c] -= 1e-6
+        fd.append((loss(forward(tp)) - loss(forward(tm))) / 2e-6)
     rel = [abs(g - f) / abs(f) for g, f in zip(grad, fd)]
-    return {"workload": f"{n}x{n} cavity, FP64 moments, {nsteps} steps, checkpoint every {seg}, "
-                        f"{n_probes} probes; gradient at (St, k_ratio, t_hot) = "
-                        f"({theta0[0]}, {theta0[1]}, {theta0[2]})",
+    return {"workload": f"{w.note}; 16 probes; gradient at (St, k_ratio, T_h) = "
+                        f"{configs.ADJ_THETA0}, target {configs.ADJ_TARGET}",
             "loss": loss(rec), "grad": [float(g) for g in grad], "central_difference": fd,
-            "fd_steps": [1e-6, 1e-6, 1e-6 * dT], "rel_error": rel,
+            "fd_step": 1e-6, "rel_error": rel, "within_1e-4": max(rel) <= 1e-4,
             "forward_ms_per_step": round(t_plain / nsteps * 1e3, 5),
             "recorded_forward_ms_per_step": round(t_fwd / nsteps * 1e3, 5),
             "backward_ms_per_step": round(t_bwd / nsteps * 1e3, 5),
             "backward_over_forward": round(t_bwd / t_plain, 3)}
 
 
-def progress(msg: str) -> None:
-    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
-
-
-def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: int = 17,
-                  n_probes: int = 16, check_n: int | None = 1024):
-    """The large adjoint workload: FP32 moment storage at n^2 (cavity of the
-    bench, Ra 1e7, St 0.1) over nsteps steps with two levels of checkpoints
-    (about nsteps^(1/3) steps apart in the inner level); the loss is the
-    mean squared error of T at seeded probes against the trajectory at St
-    0.12, T_h 1.05, k_ratio 1.5.  Then the FP32 and the FP64 adjoint of the
-    same loss at check_n^2, for the gradient check."""
+def adjoint_large(nsteps: int = 5000, seg_outer: int = 289, seg: int = 17,
+                  check_n: int | None = 1024):
+    """config5: FP32 moment storage at 8192^2 over nsteps steps with two
+    levels of checkpoints (about nsteps^(1/3) steps apart in the inner
+    level); the loss is the mean squared error of T at 16 seeded probes
+    against the trajectory at (St, k_ratio, T_h) = (0.12, 1.5, 1.05).  Then
+    the FP32 and the FP64 adjoint of the same loss at check_n^2, for the
+    gradient check."""
     import numpy as np
+    from bgk_pcm_amd import configs
     from bgk_pcm_amd.adjoint import seeded_probes
     from bgk_pcm_amd.lattice import Lattice
-    from bgk_pcm_amd.params import FP32_MOM, FP64_MOM, ROUND_CLEAN
+    from bgk_pcm_amd.params import FP32_MOM, FP64_MOM
 
     def record(base, theta, pi, pj, keep=False):
         st, kr, th = theta
-        lat = Lattice(base.replace(stefan_number=st), initialize=False, t_hot=th, k_ratio=kr)
+        lat = Lattice(base.replace(stefan_number=st), initialize=False,
+                      t_hot=configs.t_hot_of(base, th), k_ratio=kr)
         lat.initialization()
         lat.adj_setup(nsteps, seg, pi, pj, seg_outer=seg_outer)
         t0 = time.perf_counter()
@@ -355,8 +375,8 @@ def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: 
         lat.close()
         return rec
 
-    def gradient(base, pi, pj, target, theta0):
-        lat, rec, tf = record(base, theta0, pi, pj, keep=True)
+    def gradient(base, pi, pj, target):
+        lat, rec, tf = record(base, configs.ADJ_THETA0, pi, pj, keep=True)
         t0 = time.perf_counter()
         g = lat.adj_backward(2.0 * (rec - target) / rec.size)
         tb = time.perf_counter() - t0
@@ -366,62 +386,45 @@ def adjoint_large(n: int = 8192, nsteps: int = 5000, seg_outer: int = 289, seg: 
         return g, tf, tb, adj_bytes, nstates, handle_bytes
 
     out = {}
-    runs = [(n, (FP32_MOM,))] + ([(check_n, (FP32_MOM, FP64_MOM))] if check_n else [])
+    w = configs.config5()
+    runs = [(w.params.nx, (FP32_MOM,))] + ([(check_n, (FP32_MOM, FP64_MOM))] if check_n else [])
     for size, prec_list in runs:
-        base = cavity(size).replace(rounding=ROUND_CLEAN, stefan_number=0.1)
-        dT = base.t0 - base.t_in_sol
-        theta0 = (0.1, 1.0, base.t0)
-        pi, pj = seeded_probes(size, size, n_probes, seed=0)
+        base = configs.config5(size).params
+        pi, pj = seeded_probes(size, size, 16, seed=0)
         for prec in prec_list:
             b = base.replace(precision=prec)
             progress(f"adjoint {size}^2 {'FP32' if prec == FP32_MOM else 'FP64'} moments: target")
-            target = record(b, (0.12, 1.5, 273.0 + 1.05 * dT), pi, pj)
+            target = record(b, configs.ADJ_TARGET, pi, pj)
             progress(f"adjoint {size}^2: forward and backward")
-            g, tf, tb, adj_bytes, nstates, hb = gradient(b, pi, pj, target, theta0)
-            out[(size, prec)] = (g, tf, tb, adj_bytes, nstates, hb)
+            out[(size, prec)] = gradient(b, pi, pj, target)
+    n = w.params.nx
     g, tf, tb, adj_bytes, nstates, hb = out[(n, FP32_MOM)]
     # plain forward steps of the same handle type over the same steps (the
     # step gets slower as the melting front widens), for the cost ratio
     progress(f"adjoint {n}^2: plain forward steps")
-    with Lattice(cavity(n).replace(precision=FP32_MOM, rounding=ROUND_CLEAN,
-                                   stefan_number=0.1)) as lat:
+    with Lattice(w.params) as lat:
         lat.synchronize()
         t0 = time.perf_counter()
         lat.step(nsteps)
         lat.synchronize()
         t_plain = (time.perf_counter() - t0) / nsteps
-    res = {"workload": f"{n}x{n} cavity, FP32 moments, Ra 1e7, St 0.1, {nsteps} steps, "
-                        f"checkpoints every {seg_outer} and {seg} steps, {n_probes} probes",
-            "grad": [float(x) for x in g],
-            "states_stored": nstates,
-            "adjoint_device_bytes": adj_bytes, "handle_device_bytes": hb,
-            "hbm_footprint_gb": round((adj_bytes + hb) / 1e9, 2),
-            "forward_ms_per_step": round(t_plain * 1e3, 4),
-            "recorded_forward_ms_per_step": round(tf / nsteps * 1e3, 4),
-            "backward_ms_per_step": round(tb / nsteps * 1e3, 4),
-            "backward_over_forward": round(tb / nsteps / t_plain, 3)}
+    res = {"workload": f"{w.note}, checkpoints every {seg_outer} and {seg} steps, 16 probes",
+           "grad": [float(x) for x in g],
+           "states_stored": nstates,
+           "adjoint_device_bytes": adj_bytes, "handle_device_bytes": hb,
+           "hbm_footprint_gb": round((adj_bytes + hb) / 1e9, 2),
+           "forward_ms_per_step": round(t_plain * 1e3, 4),
+           "recorded_forward_ms_per_step": round(tf / nsteps * 1e3, 4),
+           "backward_ms_per_step": round(tb / nsteps * 1e3, 4),
+           "backward_over_forward": round(tb / nsteps / t_plain, 3)}
     if check_n:
         g32, g64 = out[(check_n, FP32_MOM)][0], out[(check_n, FP64_MOM)][0]
+        rel = [float(abs(a - b) / abs(b)) for a, b in zip(g32, g64)]
         res["check"] = {"size": f"{check_n}x{check_n}", "steps": nsteps,
                         "grad_fp32_moments": [float(x) for x in g32],
                         "grad_fp64_moments": [float(x) for x in g64],
-                        "rel_difference": [float(abs(a - b) / abs(b)) for a, b in zip(g32, g64)]}
+                        "rel_difference": rel, "within_1e-4": max(rel) <= 1e-4}
     return res
-
-
-def measure(p, stream, steps, warmup, kind, kernel, ext=None):
-    """MLUPS from the wall clock and roofline figures of the fused kernel
-    from HIP events on its stream (ext: physics extensions of the handle)."""
-    n = p.nx
-    kms, wms, _ = gpu_run(p, stream, steps, warmup, ext)
-    lups = p.nx * p.ny
-    nbytes = BYTES_PER_LUP[kind] * lups
-    ach = nbytes / (kms * 1e-3) / 1e9
-    return {"mlups": round(lups / (wms * 1e-3) / 1e6, 1), "ms_per_step": round(wms, 5),
-            "kernel": kernel, "kernel_ms": round(kms, 5), "achieved_gbs": round(ach, 1),
-            "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_lup": BYTES_PER_LUP[kind],
-            "algorithmic_bytes_per_launch": nbytes,
-            "traffic": traffic_from_profile(n, kind)}
 
 
 def main():
@@ -431,7 +434,8 @@ def main():
               "(the multi-GPU slab decomposition is design-only)", file=sys.stderr)
         sys.exit(2)
     import torch
-    from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, ROUND_CLEAN
+    from bgk_pcm_amd import configs
+    from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, FP64
 
     torch.cuda.set_device(0)
     if args.adjoint_large_only:
@@ -440,16 +444,16 @@ def main():
         return
     stream = torch.cuda.Stream()
     n = args.size
-    # the config's Stefan number and conductivity ratio (an extension of
-    # every handle; the reference has none)
-    p = cavity(n).replace(stefan_number=STEFAN)
-    ext = {"k_ratio": K_RATIO}
-    # the moment and FP32-population kernels follow the clean rounding of the
-    # collision; beta (forces.f90:13) is then FP64 as well
-    pm = p.replace(precision=FP32_MOM, rounding=ROUND_CLEAN)
-    pp = p.replace(precision=FP32_DEV, rounding=ROUND_CLEAN)
+    w = workload(n)
+    p, ext = w.params, w.ext
+    pm = p.replace(precision=FP32_MOM)
+    pp = p.replace(precision=FP32_DEV)
     mom_kind = "mom_eq" if p.tau_f == 1.0 else "mom"
 
+    err = None
+    if args.error_steps > 0 and not args.headline_only:
+        err = errors(w, args.error_steps)
+    progress("timed steps")
     mm = measure(pm, stream, args.steps, args.warmup, mom_kind, "k_fused_mom", ext)
     res = {
         "metric": METRIC,
@@ -465,16 +469,18 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"melting cavity {n}x{n}, Ra 1e7, Stefan {STEFAN}, solid/liquid "
-                        f"conductivity ratio {K_RATIO:g} (UTIL/input.dat.0 otherwise), "
-                        "fused time step, FP32 moment storage",
+            "workload": f"melting cavity {n}x{n}, Ra 1e7, Pr 0.02, Stefan "
+                        f"{p.stefan_number:g}, solid/liquid conductivity ratio "
+                        f"{ext['k_ratio']:g} (UTIL/input.dat.0 otherwise), fused time step, "
+                        "FP32 moment storage",
             "nx": n, "ny": n,
-            "storage": "per node rho, T-273 FP64; u, v, (1-omega_g) Pi_g^neq FP32"
-                       + ("" if mom_kind == "mom_eq" else ", (1-omega_f) Pi_f^neq FP32"),
+            "storage": "per node rho, T-273 FP64; u, v, (1-omega_f) Pi_f^neq, "
+                       "(1-omega_g) Pi_g^neq FP32",
             "arithmetic": "f64",
             "boundaries": "no-slip (type 4) on all walls",
-            "tau_f": p.tau_f, "tau_g": p.tau_g, "grav": p.grav, "rayleigh": RAYLEIGH,
-            "stefan_number": p.stefan_number, "k_ratio": K_RATIO,
+            "tau_f": p.tau_f, "tau_g": p.tau_g, "grav": p.grav, "rayleigh": 1e7,
+            "prandtl": (2 * p.tau_f - 1) / (2 * p.tau_g - 1),
+            "stefan_number": p.stefan_number, "k_ratio": ext["k_ratio"],
         },
         "roofline": {
             "bound": "hbm",
@@ -485,6 +491,7 @@ def main():
             "traffic": mm["traffic"],
             "kernel": mm["kernel"],
             "kernel_ms": mm["kernel_ms"],
+            "last_launch_extra_ms": mm["last_launch_extra_ms"],
             "bytes_per_lup": mm["bytes_per_lup"],
             "algorithmic_bytes_per_launch": mm["algorithmic_bytes_per_launch"],
         },
@@ -492,21 +499,23 @@ def main():
     res["fp32_population"] = measure(pp, stream, args.steps, args.warmup, "fp32", "k_fused32",
                                      ext)
     if not args.no_fp64:
-        res["fp64"] = measure(p, stream, max(args.steps // 2, 1), args.warmup, "fp64",
-                              "k_fused64", ext)
+        res["fp64"] = measure(p.replace(precision=FP64), stream, max(args.steps // 2, 1),
+                              args.warmup, "fp64", "k_fused64", ext)
         res["mom_over_fp64"] = round(res["value"] / res["fp64"]["mlups"], 3)
         res["fp32_population_over_fp64"] = round(
             res["fp32_population"]["mlups"] / res["fp64"]["mlups"], 3)
-    if args.error_steps > 0:
-        progress(f"FP32 moment errors, conductivity ratio {K_RATIO:g}, {args.error_steps} steps")
-        res["fp32_mom_error"] = {"k_ratio": K_RATIO,
-                                 "vs_fp64_mom": mom_error_vs_fp64_mom(pm, ext, args.error_steps)}
-        progress(f"FP32 storage errors, {args.error_steps} steps")
-        res["fp32_error"] = fp32_error(p, {"mom": pm, "fp32_population": pp}, args.error_steps,
-                                       ext)
+        res["mom_over_fp64_kernel"] = round(res["fp64"]["kernel_ms"] / mm["kernel_ms"], 3)
+        res["fp32_population_over_fp64_kernel"] = round(
+            res["fp64"]["kernel_ms"] / res["fp32_population"]["kernel_ms"], 3)
+    if err is not None:
+        res["fp32_error"] = err
+    if args.headline_only:
+        print(json.dumps(res), flush=True)
+        return
     # one GPU's share of the 8192^2 lattice over 8 GPUs (design-only sharding)
-    ps = cavity(8192).replace(ny=1026, precision=FP32_MOM, rounding=ROUND_CLEAN)
-    res["slab"] = dict(measure(ps, stream, args.steps, args.warmup, mom_kind, "k_fused_mom"),
+    ps = configs.cavity(8192, 1e7, stefan_number=p.stefan_number, rounding=p.rounding,
+                        precision=FP32_MOM, **configs.PR002).replace(ny=1026)
+    res["slab"] = dict(measure(ps, stream, args.steps, args.warmup, mom_kind, "k_fused_mom", ext),
                        workload="8192 x 1026: one GPU's rows of 8192^2 on 8 GPUs + 2 halo rows")
     if not args.no_adjoint:
         progress("adjoint 512^2")

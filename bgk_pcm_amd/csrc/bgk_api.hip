@@ -143,6 +143,8 @@ size_t carve(bgk_handle* h, char* base) {
   h->tmp = cv.plane<double>(n);
   h->teast[0] = cv.block<double>(h->ny + 2);
   h->teast[1] = cv.block<double>(h->ny + 2);
+  h->rref[0] = fp32 ? cv.block<double>(2 * ((int64_t)h->ny + 2)) : nullptr;
+  h->rref[1] = fp32 ? cv.block<double>(2 * ((int64_t)h->ny + 2)) : nullptr;
   h->partial = cv.block<double>(bgk::reduce_partial_count(h));
   h->small = cv.block<double>(h->small_elems);
   return cv.off + 256;
@@ -221,7 +223,8 @@ int launched(hipError_t e, const char* what) {
 struct FieldRef {
   double* d64 = nullptr;  // FP64 field; element stride `stride`
   float* d32 = nullptr;   // FP32 deviation member of a pair plane (stride 2)
-  double base = 0.0;
+  double base = 0.0;      // deviation base (times the reference density with
+  bool rho_table = false; // rho_table: the f members of an FP32 handle)
   int stride = 1;
 };
 
@@ -236,7 +239,8 @@ int resolve(bgk_handle* h, int field, FieldRef* r) {
     const int k = isf ? field - BGK_F0 : field - BGK_G0;
     if (fp32) {
       r->d32 = isf ? h->f32[h->cur][k] : h->g32[h->cur][k];
-      r->base = isf ? h->c.w[k] * h->c.rho0 : h->c.w[k] * h->c.g_ref;
+      r->base = isf ? h->c.w[k] : h->c.w[k] * h->c.g_ref;
+      r->rho_table = isf;
     } else {
       r->d64 = isf ? h->f[h->cur][k] : h->g[h->cur][k];
       r->stride = 2;
@@ -349,6 +353,13 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
   h->adj = nullptr;
   h->cur = 0;
   h->pc = 0;
+  h->rc = 0;
+  h->since_rebase = 0;
+  // steps between re-basings of the FP32 populations' row densities
+  // (BGK_REBASE_EVERY, 0: never); a tuning knob, every value is correct
+  h->rebase_every = 500;
+  if (const char* env = std::getenv("BGK_REBASE_EVERY"))
+    h->rebase_every = std::max(0, std::atoi(env));
   std::memset(&h->ext, 0, sizeof h->ext);
   h->ext.k_ratio = 1.0;
   // tiling of the moment kernel: BGK_MOM_ROWS (rows per strip; default
@@ -380,6 +391,14 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
   }
   h->stream = h->own_stream;
   e = hipMemsetAsync(h->pool, 0, h->pool_bytes, h->stream);
+  if (e == hipSuccess && h->rref[0]) {
+    // the reference densities start at rho0 (what an upload encodes against
+    // before any initialization)
+    const std::vector<double> r0(2 * ((size_t)h->ny + 2), p->rho0);
+    for (int b = 0; b < 2 && e == hipSuccess; ++b)
+      e = hipMemcpyAsync(h->rref[b], r0.data(), r0.size() * sizeof(double),
+                         hipMemcpyHostToDevice, h->stream);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) {
     (void)hipStreamDestroy(h->own_stream);
@@ -451,14 +470,20 @@ int bgk_upload(bgk_handle* h, int32_t field, const double* src, int64_t ld) {
   FieldRef r;
   if ((rc = resolve(h, field, &r))) return rc;
   bgk::adj_invalidate(h);
-  const bool direct = r.d64 && r.stride == 1;
+  // the stored f of an FP32 handle depend on phi through their reference
+  // density: a new phi is staged and the f re-expressed before it lands
+  const bool phi_via_tmp = h->prm.precision == BGK_FP32_DEV && field == BGK_PHI;
+  const bool direct = r.d64 && r.stride == 1 && !phi_via_tmp;
   double* dst = direct ? r.d64 : h->tmp;
   HIP_TRY(hipMemcpy2DAsync(dst, (size_t)h->pitch * sizeof(double), src, (size_t)ld * sizeof(double),
                            (size_t)(h->nx + 2) * sizeof(double), (size_t)(h->ny + 2),
                            hipMemcpyHostToDevice, h->stream),
           "hipMemcpy2DAsync(H2D)");
-  if (r.d32) HIP_TRY(bgk::launch_f64_to_dev(h, h->tmp, r.d32, 2, r.base), "f64_to_dev");
-  if (!direct && r.d64) HIP_TRY(bgk::launch_copy64(h, h->tmp, 1, r.d64, r.stride), "copy64");
+  if (r.d32) HIP_TRY(bgk::launch_f64_to_dev(h, h->tmp, r.d32, 2, r.base, r.rho_table), "f64_to_dev");
+  if (!direct && r.d64 && !phi_via_tmp)
+    HIP_TRY(bgk::launch_copy64(h, h->tmp, 1, r.d64, r.stride), "copy64");
+  if (phi_via_tmp)
+    HIP_TRY(bgk::launch_rephase32(h, h->tmp), "re-express f for the new phi");
   HIP_TRY(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
   return BGK_OK;
 }
@@ -472,7 +497,7 @@ int bgk_download(bgk_handle* h, int32_t field, double* dst, int64_t ld) {
   if ((rc = resolve(h, field, &r))) return rc;
   const double* src = r.d64;
   if (r.d32) {
-    HIP_TRY(bgk::launch_dev_to_f64(h, r.d32, 2, h->tmp, r.base), "dev_to_f64");
+    HIP_TRY(bgk::launch_dev_to_f64(h, r.d32, 2, h->tmp, r.base, r.rho_table), "dev_to_f64");
     src = h->tmp;
   } else if (r.stride != 1) {
     HIP_TRY(bgk::launch_copy64(h, r.d64, r.stride, h->tmp, 1), "copy64");
@@ -585,7 +610,7 @@ int bgk_gather(bgk_handle* h, int32_t field, int32_t n, const int32_t* i, const 
                   h->ny + 1);
   const double* plane = r.d64;
   if (r.d32) {
-    HIP_TRY(bgk::launch_dev_to_f64(h, r.d32, 2, h->tmp, r.base), "dev_to_f64");
+    HIP_TRY(bgk::launch_dev_to_f64(h, r.d32, 2, h->tmp, r.base, r.rho_table), "dev_to_f64");
     plane = h->tmp;
   } else if (r.stride != 1) {
     HIP_TRY(bgk::launch_copy64(h, r.d64, r.stride, h->tmp, 1), "copy64");

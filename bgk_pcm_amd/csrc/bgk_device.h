@@ -21,6 +21,14 @@ constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 
+// Lattice constants as literals (STEP_7/setup.f90:79-99, the same doubles
+// set_constants computes), so that kernels need no scalar registers for
+// them: w_k, cs2 and coef_k = (0.5 w_k)/(cs2 cs2).
+constexpr double kCs2 = 1.0 / 3.0;
+constexpr double kW[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
+                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+constexpr double coef_of(int k) { return (0.5 * kW[k]) / (kCs2 * kCs2); }
+
 // Lattice geometry of a handle: plane pointers address node (0,0); node
 // (i,j) is at [j*pitch + i].
 struct Geom {
@@ -128,7 +136,7 @@ __device__ __forceinline__ double omega_g_at(const Consts& c, double phi) {
   if (c.k_ratio == 1.0) return c.omega_g;
   const double fl = 0.5 * (1.0 + phi);
   const double alpha = c.alpha_l * (fl + c.k_ratio * (1.0 - fl));
-  return 1.0 / (0.5 + alpha / c.cs2);
+  return 1.0 / (0.5 + alpha / kCs2);
 }
 
 // Uniform [0,1) number of node q for the initial noise: SplitMix64 output

@@ -26,6 +26,7 @@
 #include <algorithm>
 
 #include "bgk_mom.h"
+#include "bgk_strip.h"
 
 namespace bgk {
 namespace {
@@ -45,16 +46,6 @@ struct Ring {
   typename Vec<R>::V4 b[NS][W + 2];
   typename CType<R, EQF>::T c[NS][W + 2];
   double phi[NS][W + 2];
-};
-
-// Row strips: workgroup b runs on XCD b % 8; all column segments of a strip
-// go to one XCD (its halo columns are L2 hits) and the eight XCDs work on
-// eight neighbouring strips.
-struct StripGrid {
-  int nseg;    // column segments of W
-  int rows;    // rows per strip
-  int nstrip;  // ceil(ny / rows)
-  int nmain;   // workgroups of the strips (padded to a multiple of 8 strips)
 };
 
 template <typename R, bool EQF>
@@ -134,11 +125,8 @@ __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
   __shared__ Ring<W, NS, R, EQF> L;
   const int nx = gm.nx, ny = gm.ny;
   const int64_t P = gm.pitch;
-  const int blk = (int)blockIdx.x;
-  const int kq = blk / NXCD;
-  const int strip = (kq / sg.nseg) * NXCD + blk % NXCD;
-  if (strip >= sg.nstrip) return;  // whole workgroup
-  const int seg = kq % sg.nseg;
+  int strip, seg;
+  if (!strip_of(sg, (int)blockIdx.x, &strip, &seg)) return;  // whole workgroup
   const int j0 = strip * sg.rows + 1;
   const int j1 = min(j0 + sg.rows - 1, ny);
   const int t = (int)threadIdx.x;
@@ -208,23 +196,6 @@ __device__ __forceinline__ void mom_interior(const MomIn& in, const MomOut& out,
     }
     __syncthreads();
   }
-}
-
-// Node (i, j) of the boundary-adjacent ring i in {1, Nx} or j in {1, Ny}
-// for edge index e: row 1, row Ny, then columns 1 and Nx of rows 2..Ny-1.
-__device__ __forceinline__ bool edge_node(int e, int nx, int ny, int* i, int* j) {
-  const int nrow = ny > 1 ? 2 : 1;
-  if (e < nrow * nx) {
-    *j = (e < nx) ? 1 : ny;
-    *i = e % nx + 1;
-    return true;
-  }
-  e -= nrow * nx;
-  const int ncol = nx > 1 ? 2 : 1;
-  if (ny <= 2 || e >= ncol * (ny - 2)) return false;
-  *j = e / ncol + 2;
-  *i = (e % ncol == 0) ? 1 : nx;
-  return true;
 }
 
 // The boundary-adjacent nodes, one per thread, with BC_f type 4 and BC_g
@@ -371,35 +342,6 @@ __global__ void k_initialization_mom(MomOut st, Macro m, double* phi, double* ph
   phip[q] = s0.phi;
 }
 
-// Row strips of about `target` rows, and at least one workgroup per slot
-// of the device (slots = CUs x resident workgroups per CU) on small
-// lattices, down to 2 rows per strip.  The strip count is chosen, where one
-// exists within a factor two, as a multiple of 8 (the XCDs) whose workgroups
-// fill whole rounds of the device: measured at 4096^2, 64-row strips (512
-// workgroups) run at 57 GLUPS and 96-row strips (43 strips, 1.5 rounds) at 43.
-StripGrid make_strip_grid(const bgk_handle* h, int w, int target, int slots) {
-  StripGrid s;
-  const int ny = h->ny;
-  s.nseg = (h->nx + w - 1) / w;
-  if (h->strip_rows > 0) {
-    s.rows = h->strip_rows;
-  } else {
-    int want = std::max(1, (ny + target - 1) / target);
-    if (slots > 0) want = std::max(want, (slots + s.nseg - 1) / s.nseg);
-    want = std::min(want, std::max(1, (ny + 1) / 2));
-    int pick = (want + NXCD - 1) / NXCD * NXCD;
-    for (int n = pick; n <= 2 * want && n <= ny; n += NXCD)
-      if (slots > 0 && ((int64_t)n * s.nseg) % slots == 0) {
-        pick = n;
-        break;
-      }
-    s.rows = std::max(1, (ny + pick - 1) / pick);
-  }
-  s.nstrip = (ny + s.rows - 1) / s.rows;
-  s.nmain = ((s.nstrip + NXCD - 1) / NXCD) * NXCD * s.nseg;
-  return s;
-}
-
 template <int W, int NS, typename R, bool EQF>
 hipError_t launch_mom_t(bgk_handle* h, const MomStep& st) {
   // resident workgroups per CU of this instantiation (LDS and registers)
@@ -410,9 +352,9 @@ hipError_t launch_mom_t(bgk_handle* h, const MomStep& st) {
       n = 0;
     return n;
   }();
-  const StripGrid sg = make_strip_grid(h, W, EQF ? 64 : 32, per_cu * h->n_cu);
-  const int nx = h->nx, ny = h->ny;
-  const int nedge = nx * (ny > 1 ? 2 : 1) + (ny > 2 ? (ny - 2) * (nx > 1 ? 2 : 1) : 0);
+  const StripGrid sg =
+      make_strip_grid(h->nx, h->ny, W, EQF ? 64 : 32, per_cu * h->n_cu, h->strip_rows);
+  const int nedge = edge_count(h->nx, h->ny);
   const int64_t nwg = (int64_t)sg.nmain + (nedge + W - 1) / W;
   Macro m{h->rho, h->u, h->v, h->T};
   if (st.macro[0]) m = Macro{st.macro[0], st.macro[1], st.macro[2], st.macro[3]};

@@ -26,7 +26,9 @@ struct AdjCtx;  // adjoint context (bgk_adjoint.hip)
 // f[cur], writes f[1-cur] and flips cur.  f_k and g_k of a buffer share one
 // pair plane (f at element 2q, g at 2q+1): f[b][k] points at the plane and
 // g[b][k] = f[b][k] + 1.  phi[pc] is phi and phi[1-pc] phi_prec.  FP32
-// handles hold f32/g32 = f - w*rho0, g - w*273 (T_crit) the same way.
+// handles hold f32/g32 = f - w*rho_ref, g - w*273 (T_crit) the same way,
+// rho_ref the node's reference density from the tables rref[rc] (the solid
+// and the liquid density of each row, blended by the liquid fraction).
 // Moment handles hold per node ma = (rho, T - 273) FP64, mb = (u, v, Pf_xx,
 // Pf_yy) and mc = (Pf_xy, Pg_xx, Pg_yy, Pg_xy) FP32, double-buffered by cur
 // (with omega_f = 1: mb = (u, v, Pg_xx, Pg_yy), mc = Pg_xy).
@@ -63,6 +65,13 @@ struct bgk_handle {
   int pc;
   double *rho, *u, *v, *T, *rho2, *u2, *v2, *T2;
   double* teast[2];  // averaged T(Nx, 0..Ny+1) of the previous step, ping-pong
+  // FP32 population handles: solid and liquid reference densities of rows
+  // 0..Ny+1 ([0, Ny+2) and [Ny+2, 2(Ny+2)) of rref[rc]; the other is the
+  // next table) the f members are stored against, and the re-basing cadence
+  double* rref[2];
+  int rc;
+  int since_rebase;
+  int rebase_every;
   double* tmp;       // one FP64 plane: staging for population transfers
   double* partial;   // reduction partials
   double* small;     // small device outputs (reductions, gathers)
@@ -97,11 +106,18 @@ hipError_t launch_fused_step32(bgk_handle* h, int step_parity, bool write_all);
 hipError_t launch_fused_step_mom(bgk_handle* h, int step_parity, bool write_all);
 hipError_t launch_initialization_mom(bgk_handle* h);
 
-// ---- plane conversions (bgk_fused32.hip); strides in elements
+// ---- plane conversions (bgk_fused32.hip); strides in elements.  The base
+// of a node is `base`, or with rho_table (the f members of an FP32 handle)
+// base * the node's reference density (rho0 on the halo).
 hipError_t launch_dev_to_f64(bgk_handle* h, const float* src, int sstride, double* dst,
-                             double base);
+                             double base, bool rho_table);
 hipError_t launch_f64_to_dev(bgk_handle* h, const double* src, float* dst, int dstride,
-                             double base);
+                             double base, bool rho_table);
+// FP32 handles: reset the reference-density tables to the rows' mean solid
+// and liquid densities, re-expressing the stored f; and re-express the
+// stored f for a new phi (copied into phi[pc]) before an upload of phi
+hipError_t launch_rebase32(bgk_handle* h);
+hipError_t launch_rephase32(bgk_handle* h, const double* phi_new);
 hipError_t launch_copy64(bgk_handle* h, const double* src, int sstride, double* dst, int dstride);
 // initialization of an FP32 handle (FP64 equilibria rounded to deviations)
 hipError_t launch_initialization32(bgk_handle* h);

@@ -21,12 +21,6 @@
 namespace bgk {
 namespace mom {
 
-// Lattice constants as literals (STEP_7/setup.f90:79-99), so that they need
-// no scalar registers: w_k, cs2 and coef_k = (0.5 w_k)/(cs2 cs2).
-constexpr double kCs2 = 1.0 / 3.0;
-constexpr double kW[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
-                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
-constexpr double coef_of(int k) { return (0.5 * kW[k]) / (kCs2 * kCs2); }
 constexpr double kTcrit = 273.0;  // T_crit, STEP_7/phase_field.f90:16
 
 // ---- storage traits: R = float (FP32 moments) or double (FP64 moments)

@@ -1,85 +1,270 @@
-"""The benchmark suite's smaller configurations against the oracle (the
-4096^2 and 8192^2 ones are measured by bench.py).
+"""The five benchmark configurations (bgk_pcm_amd/configs.py) at their full
+sizes and lengths.
 
- - 256^2 cavity, tau_f 0.8 with tau_g 15.5 (Pr 0.02), Stefan 0.1, 10000
-   steps.  Its Rayleigh number 1e5 needs grav = 0.124, with which STEP_7
-   itself reaches NaN before step 1000 (its buoyancy uses the absolute
-   temperature), so grav is 1e-4 as in a stable run of the same cavity.
-   The FP64 fused step against STEP_7's arithmetic (the oracle), bit for
-   bit, over the whole run; the config's cold east wall (T_c = T_in_sol)
-   and seeded initial noise (1e-3 in theta units) are extensions of the
-   moment handles and run through FP64 moment storage against the oracle
-   with the same extensions (clean rounding).
- - 2048^2 cavity with grav for Ra 1e6 (Stefan 0.1, the reference's
-   relaxation times): the FP64 fused step bit for bit against the oracle
-   for 100 of its 20000 steps (the oracle updates ~2e7 nodes per second on
-   the host); its seeded noise is an extension, as above.
+ - config 1, 256^2 x 10000 steps: the FP64 fused step against the oracle
+   run live, bit for bit (STEP_7's physics); the literal configuration (cold
+   right wall, seeded noise) on the FP64 population kernel against the
+   oracle's digest (tests/golden/long_runs.json), bit for bit, and on FP64
+   moments against the oracle within 1e-10.
+ - config 2, 2048^2 x 20000 steps: the FP64 fused step against the digest
+   of the reference program's own run (STEP_OPENMP built from its sources:
+   STEP_7's arithmetic), bit for bit after the full run, which is how the
+   reference validates a run (UTIL/RunMe_thread.sh:7); the literal
+   configuration against the oracle's digest.
+ - config 3, 4096^2 x 20000 steps: both FP32 storages against the clean FP64
+   run from the same initial state, every field within 1e-5 (relative
+   max-norm, liquid fraction for phi) after 5000, 10000 and 20000 steps.
+ - config 4, 512^2 adjoint: the gradient of the probe loss against central
+   differences of the GPU forward (step 1e-6), within 1e-4.
+ - config 5, 8192^2 FP32 forward + adjoint with two levels of checkpoints
+   fits in HBM and gives a finite gradient; at 1024^2 the FP32 adjoint is
+   within 1e-4 of the FP64 adjoint.
 """
+import hashlib
+import json
+import pathlib
+
 import numpy as np
 import pytest
 
+from bgk_pcm_amd import configs
+from bgk_pcm_amd.adjoint import seeded_probes
 from bgk_pcm_amd.lattice import Lattice
-from bgk_pcm_amd.params import FP64_MOM, ROUND_CLEAN, Params, grav_for_rayleigh
+from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, FP64, FP64_MOM, ROUND_CLEAN
 from golden_io import compare_state
 from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
 
 STATE = ["rho", "u", "v", "T", "phi"]
+LONG = json.loads((pathlib.Path(__file__).resolve().parent / "golden" /
+                   "long_runs.json").read_text())
 
 
-def cavity_256(**kw):
-    return Params(nx=256, ny=256, length=256.0, tau_f=0.8, tau_g=15.5, grav=1e-4,
-                  stefan_number=0.1, **kw)
-
-
-def rel_errors(lat, o):
+def rel_errors(got, want):
+    """Relative max-norm per field over the interior, liquid fraction
+    (1 + phi)/2 in place of phi."""
     out = {}
     for f in STATE:
-        a, b = lat.download(f)[1:-1, 1:-1], o.field(f)[1:-1, 1:-1]
+        a, b = got[f][1:-1, 1:-1], want[f][1:-1, 1:-1]
         if f == "phi":
             a, b = (1.0 + a) / 2.0, (1.0 + b) / 2.0
         out[f] = float(np.abs(a - b).max() / np.abs(b).max())
     return out
 
 
-def test_cavity_256_fp64_is_bit_exact_over_10000_steps():
-    p = cavity_256()
-    print("config 256^2, tau_f 0.8, Pr 0.02: FP64 fused, 10000 steps", flush=True)
-    o = Oracle(p)
-    o.step(10000)
-    with Lattice(p) as lat:
-        lat.step(10000)
+def digest_mismatches(lat, entry):
+    bad = []
+    for f in STATE:
+        a = np.ascontiguousarray(lat.download(f), dtype="<f8")
+        want = entry["digests"][f]
+        if hashlib.sha256(a.tobytes()).hexdigest() != want["sha256"]:
+            bad.append(f"{f}: digest differs (min {a.min():.17g} / {want['min']:.17g}, "
+                       f"max {a.max():.17g} / {want['max']:.17g}, "
+                       f"mean {a.mean():.17g} / {want['mean']:.17g})")
+    return bad
+
+
+def check_params(p, entry):
+    d = entry["params"]
+    for k in ("nx", "ny", "tau_f", "tau_g", "grav", "stefan_number", "rounding"):
+        assert d[k] == getattr(p, k), (k, d[k], getattr(p, k))
+
+
+# ------------------------------------------------------------------ config 1
+
+def test_config1_fp64_is_bit_exact_over_10000_steps():
+    w = configs.config1()
+    print("config 1, 256^2, STEP_7 physics: FP64 fused, 10000 steps", flush=True)
+    o = Oracle(w.params)
+    o.step(w.nsteps)
+    with Lattice(w.params) as lat:
+        lat.step(w.nsteps)
         got = {f: lat.download(f) for f in STATE}
     assert compare_state(got, {f: o.field(f) for f in STATE}, STATE) == []
     assert np.isfinite(got["T"]).all()
 
 
-def test_cavity_256_cold_wall_and_noise_fp64_moments():
-    """The literal config's east wall at T_c = T_in_sol and initial noise;
-    FP64 moments rebuild the populations from moments, so they agree with
-    the oracle to rounding (measured on MI355X: at most 2e-11, liquid
-    fraction), not bit for bit."""
-    p = cavity_256(rounding=ROUND_CLEAN)
-    ext = dict(t_cold=p.t_in_sol, noise_amp=1e-3, noise_seed=2024)
-    print("config 256^2 with cold east wall and noise: FP64 moments, 10000 steps", flush=True)
-    o = Oracle(p, **ext)
-    o.step(10000)
-    with Lattice(p.replace(precision=FP64_MOM), **ext) as lat:
-        lat.step(10000)
-        err = rel_errors(lat, o)
+def test_config1_literal_fp64_populations_full_run():
+    """Cold right wall and seeded noise on the FP64 population kernel:
+    equal to the oracle's digest after the 10000 steps."""
+    w = configs.config1()
+    entry = LONG["cavity256_literal"]
+    check_params(w.params, entry)
+    assert entry["extensions"] == w.ext and entry["nsteps"] == w.nsteps
+    print("config 1 literal (cold wall, noise): FP64 fused, 10000 steps", flush=True)
+    with Lattice(w.params, **w.ext) as lat:
+        lat.step(w.nsteps)
+        assert digest_mismatches(lat, entry) == []
+
+
+def test_config1_literal_fp64_moments():
+    """The same on FP64 moment storage, which rebuilds the populations from
+    moments: within 1e-10 of the oracle (clean rounding)."""
+    w = configs.config1()
+    p = w.params.replace(rounding=ROUND_CLEAN)
+    print("config 1 literal: FP64 moments, 10000 steps", flush=True)
+    o = Oracle(p, **w.ext)
+    o.step(w.nsteps)
+    with Lattice(p.replace(precision=FP64_MOM), **w.ext) as lat:
+        lat.step(w.nsteps)
+        err = rel_errors({f: lat.download(f) for f in STATE}, {f: o.field(f) for f in STATE})
     print("relative max-norm errors:", err)
     assert max(err.values()) <= 1e-10, err
 
 
-def test_cavity_2048_fp64_is_bit_exact():
-    p = Params(nx=2048, ny=2048, length=2048.0, stefan_number=0.1)
-    p = p.replace(grav=grav_for_rayleigh(p, 1e6))
-    print("config 2048^2, Ra 1e6: FP64 fused, 100 steps", flush=True)
-    o = Oracle(p)
-    o.step(100)
-    with Lattice(p) as lat:
-        lat.step(60)
-        lat.step(40)
+# ------------------------------------------------------------------ config 2
+
+def test_config2_fp64_equals_the_reference_after_20000_steps():
+    w = configs.config2()
+    entry = LONG["cavity2048_ra1e6_pr002"]
+    check_params(w.params, entry)
+    assert entry["extensions"] is None and entry["nsteps"] == w.nsteps
+    print("config 2, 2048^2 Ra 1e6 Pr 0.02: FP64 fused 20000 steps vs the reference's "
+          "digest", flush=True)
+    with Lattice(w.params) as lat:
+        lat.step(7000)
+        lat.step(13000)
+        assert digest_mismatches(lat, entry) == []
+
+
+def test_config2_literal_fp64_full_run():
+    w = configs.config2()
+    entry = LONG["cavity2048_literal"]
+    check_params(w.params, entry)
+    assert entry["extensions"] == w.ext and entry["nsteps"] == w.nsteps
+    print("config 2 literal (cold wall, noise): FP64 fused 20000 steps vs the oracle's "
+          "digest", flush=True)
+    with Lattice(w.params, **w.ext) as lat:
+        lat.step(w.nsteps)
+        assert digest_mismatches(lat, entry) == []
+
+
+def test_config2_first_steps_against_the_oracle():
+    """The first 20 steps against the oracle run live (a readable report
+    should the digest ever differ)."""
+    w = configs.config2()
+    print("config 2: FP64 fused, 20 steps vs the oracle", flush=True)
+    o = Oracle(w.params, **w.ext)
+    o.step(20)
+    with Lattice(w.params, **w.ext) as lat:
+        lat.step(20)
         got = {f: lat.download(f) for f in STATE}
     assert compare_state(got, {f: o.field(f) for f in STATE}, STATE) == []
+
+
+# ------------------------------------------------------------------ config 3
+
+MARKS = (5000, 10000, 20000)
+
+
+@pytest.fixture(scope="module")
+def config3_fp64():
+    """The clean FP64 population run of config 3 at the marks."""
+    w = configs.config3()
+    p = w.params.replace(precision=FP64)
+    out = {}
+    print("config 3, 4096^2: clean FP64 populations, 20000 steps", flush=True)
+    with Lattice(p, **w.ext) as lat:
+        done = 0
+        for m in MARKS:
+            lat.step(m - done)
+            done = m
+            out[m] = {f: lat.download(f) for f in STATE}
+    return out
+
+
+@pytest.mark.parametrize("precision", [FP32_MOM, FP32_DEV], ids=["fp32_moments",
+                                                                  "fp32_populations"])
+def test_config3_fp32_storage_within_1e5_at_every_mark(config3_fp64, precision):
+    w = configs.config3()
+    p = w.params.replace(precision=precision)
+    errs = {}
+    print(f"config 3, 4096^2: precision {precision}, 20000 steps", flush=True)
+    with Lattice(p, **w.ext) as lat:
+        done = 0
+        for m in MARKS:
+            lat.step(m - done)
+            done = m
+            errs[m] = rel_errors({f: lat.download(f) for f in STATE}, config3_fp64[m])
+            print(m, errs[m], flush=True)
+    for m, e in errs.items():
+        assert max(e.values()) <= 1e-5, (m, e)
+
+
+# ------------------------------------------------------------------ config 4
+
+def adjoint_problem(w, n_probes=16, seg_outer=0, seg=100):
+    base = w.params
+    pi, pj = seeded_probes(base.nx, base.ny, n_probes, seed=0)
+
+    def record(theta, keep=False):
+        st, kr, th = theta
+        lat = Lattice(base.replace(stefan_number=st), initialize=False,
+                      t_hot=configs.t_hot_of(base, th), k_ratio=kr)
+        lat.initialization()
+        lat.adj_setup(w.nsteps, seg, pi, pj, seg_outer=seg_outer)
+        rec = lat.adj_forward()
+        if keep:
+            return lat, rec
+        lat.close()
+        return rec
+
+    return record
+
+
+def test_config4_adjoint_matches_central_differences():
+    w = configs.config4()
+    record = adjoint_problem(w)
+    print("config 4, 512^2 adjoint: target trajectory", flush=True)
+    target = record(configs.ADJ_TARGET)
+    theta0 = np.array(configs.ADJ_THETA0)
+    lat, rec = record(theta0, keep=True)
+    print("config 4: backward", flush=True)
+    grad = lat.adj_backward(2.0 * (rec - target) / rec.size)
+    lat.close()
+    # t_hot enters in kelvin: the gradient is per kelvin, the step 1e-6 in
+    # theta units is 1e-5 K
+    dT = w.params.t0 - w.params.t_in_sol
+    loss = lambda r: float(((r - target) ** 2).mean())
+    for c, h in enumerate((1e-6, 1e-6, 1e-6)):
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[c] += h
+        tm[c] -= h
+        print(f"config 4: central difference {c}", flush=True)
+        fd = (loss(record(tp)) - loss(record(tm))) / (2 * h)
+        g = grad[c] * (dT if c == 2 else 1.0)
+        print(f"parameter {c}: adjoint {g:.12e} central difference {fd:.12e}", flush=True)
+        assert abs(g - fd) <= 1e-4 * abs(fd), (c, g, fd)
+
+
+# ------------------------------------------------------------------ config 5
+
+def test_config5_fp32_adjoint_fits_and_matches_fp64_at_1024():
+    from bgk_pcm_amd.lattice import Lattice as L
+    w = configs.config5()
+    record = adjoint_problem(w, seg_outer=289, seg=17)
+    print("config 5, 8192^2: target trajectory (5000 steps)", flush=True)
+    target = record(configs.ADJ_TARGET)
+    lat, rec = record(configs.ADJ_THETA0, keep=True)
+    adj_bytes, nstates = lat.adj_info()
+    info = lat.info()
+    print(f"config 5: {nstates} states, {(adj_bytes + info['device_bytes']) / 1e9:.1f} GB of "
+          f"{info['device_total'] / 1e9:.1f}; backward", flush=True)
+    assert adj_bytes + info["device_bytes"] <= info["device_total"]
+    g = lat.adj_backward(2.0 * (rec - target) / rec.size)
+    lat.close()
+    assert np.isfinite(g).all() and np.all(g != 0.0), g
+    grads = {}
+    for prec in (FP32_MOM, FP64_MOM):
+        w2 = configs.config5(1024)
+        w2.params = w2.params.replace(precision=prec)
+        record = adjoint_problem(w2, seg_outer=289, seg=17)
+        print(f"config 5 check, 1024^2 precision {prec}", flush=True)
+        tgt = record(configs.ADJ_TARGET)
+        lat, rec = record(configs.ADJ_THETA0, keep=True)
+        grads[prec] = lat.adj_backward(2.0 * (rec - tgt) / rec.size)
+        lat.close()
+    rel = np.abs(grads[FP32_MOM] - grads[FP64_MOM]) / np.abs(grads[FP64_MOM])
+    print("FP32 vs FP64 adjoint at 1024^2:", rel, flush=True)
+    assert rel.max() <= 1e-4, rel
