@@ -138,10 +138,9 @@ typedef struct bgk_info {
 
 /* Physics extensions (not in the reference; the defaults t_hot = 0 ->
  * T0, k_ratio = 1, east_cold = 0, noise_amp = 0, t_crit = 0 -> 273 and
- * t_ref = 0 give STEP_7's step).  k_ratio is honoured by every handle
- * (fused and per-subroutine collisions); the others by moment handles
- * (BGK_FP32_MOM, BGK_FP64_MOM) only, and other handles refuse them with
- * BGK_E_UNSUPPORTED. */
+ * t_ref = 0 give STEP_7's step).  Every handle honours every extension (the
+ * FP64 population kernels, fused and per-subroutine, bit for bit as the
+ * oracle's restatement with the same extensions). */
 typedef struct bgk_ext {
   double t_hot;       /* hot (west) wall and initial liquid temperature in
                          place of T0 (0: T0); the scale T0 - T_in_sol of

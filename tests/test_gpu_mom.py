@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from bgk_pcm_amd.lattice import BgkError, Lattice
-from bgk_pcm_amd.params import FP32_MOM, FP64_MOM, ROUND_CLEAN, Params
+from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, FP64, FP64_MOM, ROUND_CLEAN, Params
 from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -195,27 +195,18 @@ def test_critical_and_reference_temperatures_match_oracle(precision, tol, t_crit
     assert max(err.values()) <= tol, err
 
 
-def test_extensions_need_a_moment_handle():
-    """All but the conductivity ratio, which every handle carries."""
-    with Lattice(Params(nx=8, ny=8)) as lat:
-        lat.set_extensions(0.0, 1.0)  # the defaults are accepted everywhere
-        lat.set_extensions(0.0, 2.0)
-        with pytest.raises(BgkError) as e:
-            lat.set_extensions(284.0, 1.0)
-        assert e.value.code == -4
-    with Lattice(Params(nx=8, ny=8)) as lat:
-        lat.set_extensions(t_crit=273.0)  # T_crit's STEP_7 value is a default
-        for kw in (dict(t_crit=275.0), dict(t_ref=278.0)):
-            with pytest.raises(BgkError) as e:
-                lat.set_extensions(**kw)
-            assert e.value.code == -4, kw
-    with Lattice(Params(nx=8, ny=8, precision=FP64_MOM)) as lat:
-        with pytest.raises(BgkError) as e:
-            lat.set_extensions(284.0, 0.0)
-        assert e.value.code == -1
-        with pytest.raises(BgkError) as e:
-            lat.set_extensions(t_ref=-1.0)
-        assert e.value.code == -1
+def test_extensions_are_accepted_by_every_handle_and_validated():
+    """Every handle carries every extension (the FP64 and FP32 population
+    kernels included); out-of-range values are refused with BGK_E_INVALID."""
+    for prec in (FP64, FP32_DEV, FP32_MOM, FP64_MOM):
+        with Lattice(Params(nx=8, ny=8, precision=prec)) as lat:
+            lat.set_extensions(284.0, 2.0, t_cold=273.0, noise_amp=1e-3, noise_seed=3,
+                               t_crit=275.0, t_ref=278.0)
+            for bad in (dict(t_hot=284.0, k_ratio=0.0), dict(t_ref=-1.0), dict(t_crit=-2.0),
+                        dict(noise_amp=-1e-3), dict(t_cold=-5.0)):
+                with pytest.raises(BgkError) as e:
+                    lat.set_extensions(**bad)
+                assert e.value.code == -1, (prec, bad)
 
 
 @pytest.mark.parametrize("precision,tol", [(FP64_MOM, 1e-10), (FP32_MOM, 1e-6)])
