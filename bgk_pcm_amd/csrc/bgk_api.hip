@@ -572,6 +572,29 @@ int bgk_field_check(bgk_handle* h, double out[3]) {
   return nonfinite_check(out, 3, "field_check");
 }
 
+int bgk_field_check_planes(const double* rho, const double* T, const double* phi, int32_t nx,
+                           int32_t ny, int64_t pitch, void* hip_stream, double out[3]) {
+  if (!rho || !T || !phi || !out) return fail(BGK_E_INVALID, "null argument");
+  if (nx < 1 || ny < 1 || pitch < (int64_t)nx + 2 ||
+      (int64_t)(ny + 2) * pitch > ((int64_t)1 << 40))
+    return fail(BGK_E_INVALID, "Nx=%d Ny=%d pitch=%lld: need Nx, Ny >= 1 and pitch >= Nx+2", nx,
+                ny, (long long)pitch);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  if (dev >= bgk::max_scratch_devices())
+    return fail(BGK_E_UNSUPPORTED, "device %d: the scratch table holds devices 0..%d", dev,
+                bgk::max_scratch_devices() - 1);
+  const bgk::Geom g{nx, ny, pitch};
+  double s[3];
+  HIP_TRY(bgk::field_sums_planes(rho, T, phi, g, static_cast<hipStream_t>(hip_stream), s),
+          "field sums");
+  const double nn = (double)(int32_t)((uint32_t)nx * (uint32_t)ny);
+  out[0] = s[0] / nn;
+  out[1] = s[1] / nn;
+  out[2] = s[2] / nn;
+  return nonfinite_check(out, 3, "field_check");
+}
+
 int bgk_mass_check(bgk_handle* h, double out[3]) {
   int rc = check_handle(h);
   if (rc) return rc;

@@ -131,6 +131,12 @@ hipError_t launch_gather(bgk_handle* h, const double* plane, int n, const int32_
                          double* out_dev);
 
 int reduce_partial_count(const bgk_handle* h);
+// field_check's sums over the interior of three caller-owned planes, with the
+// partials in the library's static device scratch (allocates nothing); waits
+// for the result on `stream` and writes the three sums to out_host
+hipError_t field_sums_planes(const double* a, const double* b, const double* c3, const Geom& g,
+                             hipStream_t stream, double out_host[3]);
+int max_scratch_devices();
 
 // ---- adjoint (bgk_adjoint.hip), FP64 moment handles
 hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg_inner, int np,

@@ -8,6 +8,8 @@
 // and a second one-block pass.  The result is deterministic and within one
 // rounding of the exact sum, so it differs from the serial sum only by that
 // sum's own rounding error.  Counts are exact integers.
+#include <mutex>
+
 #include "bgk_internal.h"
 
 namespace bgk {
@@ -128,7 +130,55 @@ __global__ void k_gather(const double* __restrict__ plane, int64_t pitch, int n,
 
 inline int nparts(const Geom& g) { return g.ny < MAX_PARTS ? g.ny : MAX_PARTS; }
 
+// Scratch of the allocation-free entry point (field_sums_planes): the first
+// pass's partial sums, at most MAX_PARTS workgroups x 4, then the 4 results
+// of the second pass.  64 KiB per GPU, held until the process ends; every
+// call on a GPU shares it.  The first pass writes every partial the second
+// reads, so nothing needs clearing between calls.
+constexpr int kScratchSums = MAX_PARTS * 4 + 4;
+__device__ __align__(256) CSum scratch[kScratchSums];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+CSum* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (each GPU holds its own
+// copy of the array), looked up once per device.  The array's name is only a
+// host-side stand-in: host code passes this address, never the name.
+hipError_t scratch_of_device(int dev, CSum** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[dev] = static_cast<CSum*>(p);
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
+
 }  // namespace
+
+int max_scratch_devices() { return kMaxDevices; }
+
+hipError_t field_sums_planes(const double* a, const double* b, const double* c3, const Geom& g,
+                             hipStream_t stream, double out_host[3]) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  CSum* part = nullptr;
+  if ((e = scratch_of_device(dev, &part)) != hipSuccess) return e;
+  double* res = reinterpret_cast<double*>(part + MAX_PARTS * 4);
+  const int np = nparts(g);
+  hipLaunchKernelGGL(k_sum3, dim3(np), dim3(RB), 0, stream, a, b, c3, g, part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, stream, part, np, 3, res);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(out_host, res, 3 * sizeof(double), hipMemcpyDeviceToHost, stream)) !=
+      hipSuccess)
+    return e;
+  return hipStreamSynchronize(stream);
+}
 
 int reduce_partial_count(const bgk_handle*) { return MAX_PARTS * 4 * 2; }
 

@@ -91,6 +91,8 @@ SIGNATURES = {
     "bgk_moments": [_H],
     "bgk_step": [_H, ctypes.c_int32],
     "bgk_field_check": [_H, _D],
+    "bgk_field_check_planes": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                               ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, _D],
     "bgk_mass_check": [_H, _D],
     "bgk_diag_sums": [_H, _D],
     "bgk_gather": [_H, ctypes.c_int32, ctypes.c_int32, _I32P, _I32P, _D],

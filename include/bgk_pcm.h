@@ -225,6 +225,22 @@ int bgk_step(bgk_handle* h, int32_t nsteps);
  * behaviour ignores this code. */
 int bgk_field_check(bgk_handle* h, double out[3]);
 
+/* Allocation-free variant of bgk_field_check.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc),
+ * which cannot hold a handle either.  The same means { <rho>, <T>, <phi> }
+ * over the interior, of three device planes the caller owns (node (i,j) at
+ * plane[i + pitch*j], i = 0..Nx+1, j = 0..Ny+1, pitch >= Nx+2), computed on
+ * the current device on `hip_stream` (NULL: the null stream), which this call
+ * then waits for.  The partial sums live in the library's static device
+ * scratch (64 KiB per GPU, any lattice size: the first pass uses at most 1024
+ * workgroups); it calls no hipMalloc/hipFree.  Every call on a GPU shares
+ * that scratch: calls that overlap on one GPU (other threads, other streams)
+ * race and must be serialised by the caller.  BGK_E_NONFINITE as
+ * bgk_field_check. */
+int bgk_field_check_planes(const double* rho, const double* T, const double* phi, int32_t nx,
+                           int32_t ny, int64_t pitch, void* hip_stream, double out[3]);
+
 /* STEP_7/mass_check.f90:8-32: out = { mass (percent defect),
  * <rho>, <T> } over the interior; BGK_E_NONFINITE as bgk_field_check. */
 int bgk_mass_check(bgk_handle* h, double out[3]);
