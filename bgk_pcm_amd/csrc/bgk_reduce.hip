@@ -5,6 +5,7 @@
 // Here every block owns a fixed set of rows, every thread a fixed set of
 // columns, and each accumulates a compensated (Neumaier) FP64 sum; the
 // (sum, error) pairs are then combined by error-free TwoSum in a fixed tree
+// (cross-lane shuffles within each wavefront, then the wavefronts in order)
 // and a second one-block pass.  The result is deterministic and within one
 // rounding of the exact sum, so it differs from the serial sum only by that
 // sum's own rounding error.  Counts are exact integers.
@@ -40,21 +41,33 @@ __device__ __forceinline__ CSum cmerge(CSum a, CSum b) {
   return CSum{s, a.c + b.c + err};
 }
 
+// Workgroup reduction of N compensated sums: a butterfly within each
+// 64-wide wavefront (cross-lane shuffles, no LDS), then the RB/64 wavefront
+// results combined in wave order by the first thread.  Fixed order:
+// deterministic.  The result is valid in thread 0.
 template <int N>
-__device__ __forceinline__ void block_reduce(CSum (&v)[N], CSum (*lds)[RB]) {
+__device__ __forceinline__ void block_reduce(CSum (&v)[N], CSum (*lds)[RB / 64]) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < N; ++k) lds[k][t] = v[k];
-  __syncthreads();
-  for (int s = RB / 2; s > 0; s >>= 1) {
-    if (t < s) {
+  for (int k = 0; k < N; ++k)
 #pragma unroll
-      for (int k = 0; k < N; ++k) lds[k][t] = cmerge(lds[k][t], lds[k][t + s]);
+    for (int off = 32; off > 0; off >>= 1) {
+      CSum o;
+      o.s = __shfl_down(v[k].s, off, 64);
+      o.c = __shfl_down(v[k].c, off, 64);
+      v[k] = cmerge(v[k], o);
     }
-    __syncthreads();
-  }
+  if ((t & 63) == 0)
 #pragma unroll
-  for (int k = 0; k < N; ++k) v[k] = lds[k][0];
+    for (int k = 0; k < N; ++k) lds[k][t >> 6] = v[k];
+  __syncthreads();
+  if (t == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      CSum a = lds[k][0];
+      for (int w = 1; w < RB / 64; ++w) a = cmerge(a, lds[k][w]);
+      v[k] = a;
+    }
 }
 
 // Sums of three planes over the interior (rho, T, phi).
@@ -62,7 +75,7 @@ __global__ __launch_bounds__(RB) void k_sum3(const double* __restrict__ a,
                                              const double* __restrict__ b,
                                              const double* __restrict__ c3, Geom gm,
                                              CSum* __restrict__ part) {
-  __shared__ CSum lds[3][RB];
+  __shared__ CSum lds[3][RB / 64];
   CSum acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   for (int j = blockIdx.x + 1; j <= gm.ny; j += gridDim.x) {
     const int64_t r = (int64_t)j * gm.pitch;
@@ -84,7 +97,7 @@ __global__ __launch_bounds__(RB) void k_sum3(const double* __restrict__ a,
 __global__ __launch_bounds__(RB) void k_diag(const double* __restrict__ T,
                                              const double* __restrict__ phi, Geom gm,
                                              double tin, double dT, CSum* __restrict__ part) {
-  __shared__ CSum lds[4][RB];
+  __shared__ CSum lds[4][RB / 64];
   CSum acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   long long cnt = 0, isum = 0;
   const double dny = (double)gm.ny;
@@ -112,7 +125,7 @@ __global__ __launch_bounds__(RB) void k_diag(const double* __restrict__ T,
 // Second pass: combine nparts partials of nk sums (stride 4).
 __global__ __launch_bounds__(RB) void k_final(const CSum* __restrict__ part, int nparts, int nk,
                                               double* __restrict__ out) {
-  __shared__ CSum lds[4][RB];
+  __shared__ CSum lds[4][RB / 64];
   CSum acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   for (int p = threadIdx.x; p < nparts; p += RB)
     for (int k = 0; k < 4; ++k) acc[k] = cmerge(acc[k], part[p * 4 + k]);

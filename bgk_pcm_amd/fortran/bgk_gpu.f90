@@ -149,6 +149,10 @@ module bgk_gpu
   type(bgk_params_t), save :: prm
   real(c_double), save :: kvisc, alpha, tin
   real(c_double), save :: mass = 0.0d0
+  ! the time step counter of the driver loop (module shared's `it`,
+  ! STEP_7/shared.f90), read by the argument-less diagnostics below as the
+  ! reference's subroutines read it (STEP_7/main.f90:72-82, 90)
+  integer, save :: it = 0
 
 contains
 
@@ -335,8 +339,7 @@ contains
   end subroutine gather5
 
   ! STEP_7/field_check.f90:11-24 -> fort.64
-  subroutine field_check(it)
-    integer, intent(in) :: it
+  subroutine field_check()
     real(c_double) :: m(3)
     call check(c_bgk_field_check(handle, m), 'bgk_field_check')
     write(64, 1002) it, m(1), m(2), m(3)
@@ -345,8 +348,7 @@ contains
   end subroutine field_check
 
   ! STEP_7/mass_check.f90:8-32 -> fort.64 and stdout; sets mass
-  subroutine mass_check(it)
-    integer, intent(in) :: it
+  subroutine mass_check()
     real(c_double) :: m(3)
     call check(c_bgk_mass_check(handle, m), 'bgk_mass_check')
     mass = m(1)
@@ -356,8 +358,7 @@ contains
   end subroutine mass_check
 
   ! STEP_7/probe.f90:7-15 -> fort.666
-  subroutine probe(it)
-    integer, intent(in) :: it
+  subroutine probe()
     integer(c_int32_t) :: ii(1), jj(1)
     real(c_double) :: vals(1, 5)
     ii(1) = prm%nx / 2
@@ -368,8 +369,7 @@ contains
   end subroutine probe
 
   ! STEP_7/prof_x.f90:7-17 -> fort.667
-  subroutine prof_x(it)
-    integer, intent(in) :: it
+  subroutine prof_x()
     integer(c_int32_t), allocatable :: ii(:), jj(:)
     real(c_double), allocatable :: vals(:, :)
     integer :: i
@@ -390,8 +390,7 @@ contains
   end subroutine prof_x
 
   ! STEP_7/prof_y.f90:7-17 -> fort.668
-  subroutine prof_y(it)
-    integer, intent(in) :: it
+  subroutine prof_y()
     integer(c_int32_t), allocatable :: ii(:), jj(:)
     real(c_double), allocatable :: vals(:, :)
     integer :: j
@@ -413,8 +412,7 @@ contains
 
   ! STEP_7/diag.f90:15-131: non-dimensional numbers from the device sums
   ! -> fort.69, non-dim_bulk_numbers.dat (unit 164, opened by the driver)
-  subroutine diag(it)
-    integer, intent(in) :: it
+  subroutine diag()
     real(c_double) :: s(4)
     real(8) :: beta_gras, dT, gras, pr, ray, fou, theta, theta1, theta2, nus, &
                nus_comp, nus_comp_2, melt, front, front_exp
@@ -458,8 +456,7 @@ contains
   end subroutine diag
 
   ! STEP_7/out2d.f90:8-65: legacy ASCII VTK of rho, T, phi, (u,v,0)
-  subroutine out2d(it)
-    integer, intent(in) :: it
+  subroutine out2d()
     character(len=25) :: fname
     real(c_double), allocatable :: a(:, :), b(:, :)
     integer :: un, i, l
@@ -517,8 +514,7 @@ contains
   ! Split binary VTK files of STEP_CUF (out2d.phi.f90, out2d.rho.f90,
   ! out2d.temp.f90, out2d.vel.f90): ASCII header, then the interior nodes
   ! (j outer, i inner) as big-endian real(8) stream records.
-  subroutine out2d_bin(it)
-    integer, intent(in) :: it
+  subroutine out2d_bin()
     real(c_double), allocatable :: a(:, :), b(:, :)
     allocate(a(0:prm%nx+1, 0:prm%ny+1), b(0:prm%nx+1, 0:prm%ny+1))
     call download(F_PHI, a)
@@ -594,8 +590,7 @@ contains
   ! which the reference's restart does not keep (its dump_in restores no
   ! phi, so a run restarted from it melts from phi = 0).  A reader of the
   ! reference format reads the first part unchanged.
-  subroutine dump_out(it)
-    integer, intent(in) :: it
+  subroutine dump_out()
     real(c_double), allocatable :: a(:, :, :)
     integer :: un, i, l, k
     allocate(a(0:prm%nx+1, 0:prm%ny+1, 23))
@@ -629,8 +624,7 @@ contains
 
   ! Restart from dumpFile.dat (dump_out above; STEP_7/dump_in.f90 reads the
   ! same first part).  Stops when the phi block is missing.
-  subroutine dump_in(it)
-    integer, intent(out) :: it
+  subroutine dump_in()
     real(c_double), allocatable :: a(:, :, :)
     integer :: un, i, l, k, ios
     allocate(a(0:prm%nx+1, 0:prm%ny+1, 23))

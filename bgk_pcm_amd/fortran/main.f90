@@ -31,7 +31,7 @@ program bgk_thermal_gpu
   implicit none
 
   type(bgk_params_t) :: p
-  integer :: it, it0, iter, deltaOut, itOut, isignal, ev, s, k, nargs
+  integer :: it0, iter, deltaOut, itOut, isignal, ev, s, k, nargs
   integer :: icountT0, icountT1, icount_rate, icount_max, icountL0, icountL1
   real(8) :: time_inn_loop, mem_mb, dev_mb
   logical :: per_stage, bin_vtk, dump_at_end
@@ -91,7 +91,7 @@ program bgk_thermal_gpu
      write(150,*) '===================='
      close(150)
   else
-     call dump_in(it)
+     call dump_in
      it0 = it + 1
   end if
 
@@ -99,9 +99,9 @@ program bgk_thermal_gpu
   if (it == 0) then
      itOut = deltaOut
      if (bin_vtk) then
-        call out2d_bin(it)
+        call out2d_bin
      else
-        call out2d(it)
+        call out2d
      end if
   else
      itOut = it + deltaOut
@@ -141,11 +141,11 @@ program bgk_thermal_gpu
      it = ev
 
      if (mod(it, 1000) == 0) then
-        call diag(it)
-        call probe(it)
-        call prof_x(it)
-        call prof_y(it)
-        call field_check(it)
+        call diag
+        call probe
+        call prof_x
+        call prof_y
+        call field_check
         open(unit=150, file='log.dat', position='append')
         write(150, "(I8,A,F10.4)") it, ' ', mass
         close(150)
@@ -153,9 +153,9 @@ program bgk_thermal_gpu
 
      if (it == itOut .or. it == iter) then
         if (bin_vtk) then
-           call out2d_bin(it)
+           call out2d_bin
         else
-           call out2d(it)
+           call out2d
         end if
         open(unit=160, file='Nusselt.dat')
         do k = 1, 40
@@ -178,7 +178,8 @@ program bgk_thermal_gpu
   call synchronize()
   call SYSTEM_CLOCK(icountT1, icount_rate, icount_max)
   time_inn_loop = real(icountT1 - icountT0) / (icount_rate)
-  if (dump_at_end) call dump_out(iter)
+  it = iter
+  if (dump_at_end) call dump_out
   call memory_report(mem_mb, dev_mb)
   write(6,*) "=========================================="
   write(6,*) "INFO: Run completed"

@@ -20,6 +20,11 @@ Each fixture holds:
   fields      names of the F stored fields (a subset of the 46)
   text_<fn>   text files written by the reference diagnostics
 
+thot_pin_32x24.npz pins the hot-wall extension (t_hot) by a reduction to the
+reference: at grav = 0 the reference run with T0' = 293, Stefan' = 0.25 must
+equal the oracle at T0 = 283, Stefan 0.125, t_hot = 293 (`oracle_side`),
+bit for bit (`python tests/golden/make_golden.py thot`).
+
 formats_cav24x20.npz holds what the reference's output routines write
 after 50 steps of a 24x20 cavity (oracle/_ref/vtkbin/ref_vtkbin): the split
 binary VTK files of STEP_CUF as bytes (bin_<name>, uint8) and STEP_7's
@@ -142,6 +147,29 @@ def formats_fixture():
     return path
 
 
+# The hot-wall extension pinned by a reduction to the reference: with grav
+# = 0 the buoyancy vanishes, and the oracle at (T0, t_hot, Stefan) equals
+# the reference at (T0' = t_hot, Stefan' = Stefan (t_hot - T_in_sol) / (T0 -
+# T_in_sol)) - the same liquid and wall temperature, the same latent-heat
+# factor (T0 - T_in_sol)/Stefan (80 here, exact in both) - bit for bit.
+THOT_CASE = ("thot_pin_32x24", Params(nx=32, ny=24, length=24.0, grav=0.0, t0=293.0,
+                                      stefan_number=0.25), 300,
+             {"t0": 283.0, "stefan_number": 0.125, "t_hot": 293.0})
+
+
+def thot_fixture():
+    name, p, nsteps, oracle_side = THOT_CASE
+    r = run_ref(p, nsteps, 1, 0)
+    idx = [FIELD_NAMES.index(f) for f in STATE]
+    data = {"params": np.array(json.dumps(asdict(p))), "nsteps": np.array(nsteps),
+            "fields": np.array(STATE), "init": r["init"][idx], "final": r["final"][idx],
+            "oracle_side": np.array(json.dumps(oracle_side))}
+    path = HERE / f"{name}.npz"
+    np.savez_compressed(path, **data)
+    print(f"{name}: {path.stat().st_size / 1024:.0f} KiB")
+    return path
+
+
 LONG = HERE / "long_runs.json"
 
 
@@ -212,6 +240,9 @@ def main() -> None:
     if len(sys.argv) > 1 and sys.argv[1] == "formats":
         update_manifest(formats_fixture())
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "thot":
+        update_manifest(thot_fixture())
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "long":
         long_fixture(sys.argv[2:] or None)
         return
@@ -245,6 +276,8 @@ def main() -> None:
     path = driver_fixture()
     manifest.append(f"{hashlib.sha256(path.read_bytes()).hexdigest()}  {path.name}")
     path = formats_fixture()
+    manifest.append(f"{hashlib.sha256(path.read_bytes()).hexdigest()}  {path.name}")
+    path = thot_fixture()
     manifest.append(f"{hashlib.sha256(path.read_bytes()).hexdigest()}  {path.name}")
     (HERE / "MANIFEST.sha256").write_text("\n".join(manifest) + "\n")
 

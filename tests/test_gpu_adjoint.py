@@ -69,6 +69,65 @@ def test_adjoint_equals_autograd_of_the_cpu_reference(tau_f, seg, k_ratio):
     np.testing.assert_allclose(g, g_ref, rtol=1e-7, atol=0)
 
 
+@pytest.mark.parametrize("nx,ny,re,seg", [(30, 24, 3.0, 9), (150, 20, 0.0, 11),
+                                           (200, 12, 2.0, 25)],
+                         ids=["re3", "three_tile_columns", "re2_four_tile_columns"])
+def test_adjoint_with_inflow_velocity_and_tile_seams(nx, ny, re, seg):
+    """u0 = Re nu / length != 0 puts constant ghost velocities into the
+    thermal boundary equilibria (BC_g.f90:9-31) that the recomputed steps
+    must take from the recorded run; Nx > 128 puts two or three seams of the
+    adjoint kernel's 64-column tiles between probes (columns 64/65, 128/129,
+    192/193 are probed), against autograd of the CPU reference."""
+    p = Params(nx=nx, ny=ny, length=float(ny), grav=2e-3, rounding=ROUND_CLEAN, re=re,
+               stefan_number=0.1)
+    nsteps = 80
+    # probes near the hot wall and the front (the Stefan number and t_hot act
+    # there) and on both sides of every tile seam (the conductivity ratio
+    # acts in the whole solid, so its gradient carries the seam adjoints)
+    pi, pj = [2, 4, 3], [5, 7, 9]
+    for col in (64, 65, 128, 129, 192, 193):
+        if col <= nx and (col, 2 + col % 3) not in zip(pi, pj):
+            pi.append(col)
+            pj.append(2 + col % 3)
+    theta0 = (0.1, 1.3, 283.5)
+    target = probe_record(p, torch.tensor([0.12, 1.2, 283.2], dtype=torch.float64), pi, pj,
+                          nsteps).numpy()
+    print(f"adjoint vs autograd: {nx}x{ny}, Re {re}, {nsteps} steps, seg_len {seg}", flush=True)
+    th = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    rec_ref = probe_record(p, th, pi, pj, nsteps)
+    ((rec_ref - torch.from_numpy(target)) ** 2).mean().backward()
+    g_ref = th.grad.numpy()
+    with Lattice(p.replace(precision=FP64_MOM, stefan_number=theta0[0]), initialize=False,
+                 t_hot=theta0[2], k_ratio=theta0[1]) as lat:
+        lat.initialization()
+        lat.adj_setup(nsteps, seg, pi, pj)
+        rec = lat.adj_forward()
+        g = lat.adj_backward(2.0 * (rec - target) / rec.size)
+    r = rec_ref.detach().numpy()
+    assert np.abs(rec - r).max() <= 1e-11 * np.abs(r).max()
+    print("gradient GPU", g, "autograd", g_ref)
+    np.testing.assert_allclose(g, g_ref, rtol=1e-7, atol=0)
+
+
+def test_adjoint_misuse_is_reported():
+    """A backward pass needs a recorded forward that matches the current
+    state and constants; repeated probe nodes are refused."""
+    p = Params(nx=20, ny=16, length=16.0, precision=FP64_MOM, rounding=ROUND_CLEAN)
+    with Lattice(p) as lat:
+        lat.adj_setup(10, 5, [3, 4], [5, 6])
+        with pytest.raises(BgkError) as e:
+            lat.adj_backward(np.zeros((10, 2)))
+        assert e.value.code == -1 and "bgk_adj_forward" in str(e.value)
+        lat.adj_forward()
+        lat.step(1)  # the state moved on: the record no longer matches it
+        with pytest.raises(BgkError) as e:
+            lat.adj_backward(np.zeros((10, 2)))
+        assert e.value.code == -1
+        with pytest.raises(BgkError) as e:
+            lat.adj_setup(10, 5, [3, 4, 3], [5, 6, 5])
+        assert e.value.code == -1 and "share" in str(e.value)
+
+
 def test_adjoint_matches_central_differences():
     p = Params(nx=64, ny=64, length=64.0, grav=1e-3, rounding=ROUND_CLEAN, stefan_number=0.1)
     nsteps, seg = 400, 40

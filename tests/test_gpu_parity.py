@@ -213,3 +213,21 @@ def test_extensions_are_bit_exact_on_both_fp64_paths(rounding, ext_name):
     want = {f: o.field(f) for f in STATE + POPS}
     assert compare_state(got, want, STATE) == []
     assert compare_interior(got, want, POPS) == []
+
+
+def test_hot_wall_extension_reduces_to_the_reference_on_the_gpu():
+    """The FP64 fused step with t_hot = 293 at T0 = 283, Stefan 0.125 and
+    grav = 0 equals the reference run at T0' = 293, Stefan' = 0.25 bit for
+    bit (tests/golden/thot_pin_32x24.npz; see the oracle's test of the same
+    reduction)."""
+    import json
+    g = Golden("thot_pin_32x24")
+    side = json.loads(str(g.d["oracle_side"]))
+    p = g.params.replace(t0=side["t0"], stefan_number=side["stefan_number"])
+    print("t_hot reduction 32x24, 300 steps", flush=True)
+    with Lattice(p, t_hot=side["t_hot"]) as lat:
+        lat.step(g.nsteps)
+        got = gpu_state(lat, STATE + POPS)
+    want = g.snap("final")
+    assert compare_state(got, want, STATE) == []
+    assert compare_interior(got, want, POPS) == []

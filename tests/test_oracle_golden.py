@@ -111,3 +111,24 @@ def test_diagnostics_text(name):
     keys = [1, 2, 3, 4, 5, 8, 9, 10, 11] + ([6, 7] if valid else [])
     for k in keys:
         assert _as_fortran_e(out[k], 6) == vals[k], (k, out[k], vals[k])
+
+
+def test_hot_wall_extension_reduces_to_the_reference():
+    """The oracle's t_hot extension, pinned by a reduction: with grav = 0
+    (no buoyancy, so beta's dependence on T0 drops out) the oracle at T0 =
+    283, Stefan 0.125, t_hot = 293 equals the reference run at T0' = 293,
+    Stefan' = 0.25 (the same latent-heat factor (T0 - T_in_sol)/Stefan = 80,
+    exact in both) bit for bit, initial state and after 300 steps
+    (tests/golden/thot_pin_32x24.npz)."""
+    import json
+    g = Golden("thot_pin_32x24")
+    side = json.loads(str(g.d["oracle_side"]))
+    p = g.params.replace(t0=side["t0"], stefan_number=side["stefan_number"])
+    o = Oracle(p, t_hot=side["t_hot"])
+    assert compare_state(state_of(o, g.fields), g.snap("init"), g.fields) == []
+    o.step(g.nsteps)
+    assert compare_state(state_of(o, g.fields), g.snap("final"), g.fields) == []
+    # and the extension is not the identity: T0 = 283 without t_hot differs
+    o2 = Oracle(p)
+    o2.step(g.nsteps)
+    assert not np.array_equal(o2.field("T"), g.snap("final")["T"])
