@@ -64,8 +64,10 @@ __device__ __forceinline__ void plane_set(const AdjPlanes& L, int p, int64_t q, 
 enum { L_RHO, L_JX, L_JY, L_FXX, L_FYY, L_FXY, L_DT, L_GXX, L_GYY, L_GXY, L_KP, L_KM, L_PC };
 
 struct AdjArgs {
-  // forward: state after step n (gather derivatives) and the inputs of step n
-  StatePtrs sn, sprev;
+  // the state before step n (the inputs of step n); the state after it, whose
+  // moments the gather of step n+1's pulls differentiates at, is recomputed
+  // from it in the kernel (core = 1) or is this state itself (core = 0)
+  StatePtrs sprev;
   const double* uh;  // macroscopic u, v planes: their halo columns are the ghost velocities
   const double* vh;
   AdjPlanes next;    // adjoint of step n+1 (read)
@@ -101,8 +103,26 @@ struct AdjArgs {
 constexpr int BX = 64, WX = BX + 2;
 template <typename R, bool EQF>
 constexpr int tile_rows() {
-  return (sizeof(R) == sizeof(float) && !EQF) ? 8 : 4;
+  return sizeof(R) == sizeof(float) ? 8 : 4;
 }
+
+// Window types of the state's FP64 members (rho, T - T_crit and phi): the
+// FP32 kernel evaluates in FP32, so its window holds them rounded once, in
+// 12 bytes instead of 24.
+template <typename R>
+struct WinT {
+  using A2 = double2;
+  using P = double;
+  static __device__ __forceinline__ A2 a2(const double2 v) { return v; }
+};
+template <>
+struct WinT<float> {
+  using A2 = float2;
+  using P = float;
+  static __device__ __forceinline__ A2 a2(const double2 v) {
+    return make_float2((float)v.x, (float)v.y);
+  }
+};
 
 template <bool EQF>
 struct LSlots {
@@ -117,19 +137,19 @@ template <typename R, bool EQF>
 struct Win {
   static constexpr int WY = tile_rows<R, EQF>() + 2;
   R lam[LSlots<EQF>::N][WY][WX];  // as stored: every read widens exactly
-  double2 a[WY][WX];
+  typename WinT<R>::A2 a[WY][WX];
   typename Vec<R>::V4 b[WY][WX];
   typename CType<R, EQF>::T c[WY][WX];
-  double phi[WY][WX];
+  typename WinT<R>::P phi[WY][WX];
 };
 
 // Adjoint of population k as pulled by the node at window (x, y), from that
 // node's adjoint of its pulled moments (PullSums is linear in the
 // populations).
 template <typename R, bool EQF>
-__device__ __forceinline__ double lam_f(const Win<R, EQF>& W, int k, int x, int y) {
+__device__ __forceinline__ R lam_f(const Win<R, EQF>& W, int k, int x, int y) {
   using S = LSlots<EQF>;
-  double r = W.lam[S::RHO][y][x];
+  R r = W.lam[S::RHO][y][x];
   if constexpr (EQF) {
     if (CX[k] != 0) r += CX[k] > 0 ? W.lam[S::JX][y][x] : -W.lam[S::JX][y][x];
     if (CY[k] != 0) r += CY[k] > 0 ? W.lam[S::JY][y][x] : -W.lam[S::JY][y][x];
@@ -144,67 +164,94 @@ __device__ __forceinline__ double lam_f(const Win<R, EQF>& W, int k, int x, int 
   return r;
 }
 template <typename R, bool EQF>
-__device__ __forceinline__ double lam_g(const Win<R, EQF>& W, int k, int x, int y) {
+__device__ __forceinline__ R lam_g(const Win<R, EQF>& W, int k, int x, int y) {
   using S = LSlots<EQF>;
-  double r = W.lam[S::DT][y][x];
+  R r = W.lam[S::DT][y][x];
   if (CX[k] != 0) r += W.lam[S::GXX][y][x];
   if (CY[k] != 0) r += W.lam[S::GYY][y][x];
   if (CX[k] * CY[k] != 0) r += CX[k] * CY[k] > 0 ? W.lam[S::GXY][y][x] : -W.lam[S::GXY][y][x];
   return r;
 }
 
-// Stored moments of a node in FP64 (EQF: Pf = 0).
+// Stored moments of a node (EQF: Pf = 0), or their adjoints, in the
+// arithmetic type A.
+template <typename A>
 struct Mom {
-  double rho, th, u, v, pf[3], pg[3];
+  A rho, th, u, v, pf[3], pg[3];
 };
 
-template <typename R, bool EQF>
-__device__ __forceinline__ Mom load_mom(const StatePtrs& s, int64_t q) {
-  Mom m;
-  const double2 a = s.a[q];
-  const typename Vec<R>::V4 b = b_at<R, EQF>(s.b, q);
-  m.rho = a.x;
-  m.th = a.y;
-  m.u = (double)b.x;
-  m.v = (double)b.y;
+// The moments a forward step stores for a node (bgk_fused_mom.hip's
+// update_node), widened back to FP64: rho and T - T_crit as computed, u, v
+// and the scaled non-equilibrium moments rounded to the storage type R.
+template <typename R, bool EQF, typename A>
+__device__ __forceinline__ Mom<A> stored_mom(const Consts& c, const CoreT<A>& o) {
+  Mom<A> m;
+  m.rho = o.rho;
+  m.th = o.dT;
+  m.u = (A)(R)o.u;
+  m.v = (A)(R)o.v;
+  const A omg1 = A(1) - o.omega_g;
+  m.pg[0] = (A)(R)(omg1 * o.qgxx);
+  m.pg[1] = (A)(R)(omg1 * o.qgyy);
+  m.pg[2] = (A)(R)(omg1 * o.qgxy);
   if constexpr (EQF) {
-    m.pf[0] = m.pf[1] = m.pf[2] = 0.0;
-    m.pg[0] = (double)b.z;
-    m.pg[1] = (double)b.w;
-    m.pg[2] = (double)c_at<R, EQF>(s.c, q);
+    m.pf[0] = m.pf[1] = m.pf[2] = A(0);
   } else {
-    const typename Vec<R>::V4 cc = c_at<R, EQF>(s.c, q);
-    m.pf[0] = (double)b.z;
-    m.pf[1] = (double)b.w;
-    m.pf[2] = (double)cc.x;
-    m.pg[0] = (double)cc.y;
-    m.pg[1] = (double)cc.z;
-    m.pg[2] = (double)cc.w;
+    const A omf1 = A(1) - (A)c.omega_f;
+    m.pf[0] = (A)(R)(omf1 * o.qfxx);
+    m.pf[1] = (A)(R)(omf1 * o.qfyy);
+    m.pf[2] = (A)(R)(omf1 * o.qfxy);
+  }
+  return m;
+}
+
+// Moments of a node held in the window (the state before the step).
+template <typename R, bool EQF, typename A, typename A2>
+__device__ __forceinline__ Mom<A> win_mom(const A2& a, const typename Vec<R>::V4& b,
+                                          const typename CType<R, EQF>::T& cc) {
+  Mom<A> m;
+  m.rho = (A)a.x;
+  m.th = (A)a.y;
+  m.u = (A)b.x;
+  m.v = (A)b.y;
+  if constexpr (EQF) {
+    m.pf[0] = m.pf[1] = m.pf[2] = A(0);
+    m.pg[0] = (A)b.z;
+    m.pg[1] = (A)b.w;
+    m.pg[2] = (A)cc;
+  } else {
+    m.pf[0] = (A)b.z;
+    m.pf[1] = (A)b.w;
+    m.pf[2] = (A)cc.x;
+    m.pg[0] = (A)cc.y;
+    m.pg[1] = (A)cc.z;
+    m.pg[2] = (A)cc.w;
   }
   return m;
 }
 
 // d(H_k : P)/dP for P = (xx, yy, xy)
-__device__ __forceinline__ void h_coef(int k, double* hx, double* hy, double* hxy) {
-  *hx = (CX[k] != 0 ? 1.0 : 0.0) - kCs2;
-  *hy = (CY[k] != 0 ? 1.0 : 0.0) - kCs2;
-  *hxy = 2.0 * CX[k] * CY[k];
+template <typename A>
+__device__ __forceinline__ void h_coef(int k, A* hx, A* hy, A* hxy) {
+  *hx = (CX[k] != 0 ? A(1) : A(0)) - A(kCs2);
+  *hy = (CY[k] != 0 ? A(1) : A(0)) - A(kCs2);
+  *hxy = A(2 * CX[k] * CY[k]);
 }
 
 // Accumulate into lm the adjoint of the node's moments from the adjoints lf,
 // lg of its populations f_k and gd_k (the transposed rebuild Jacobian).
-__device__ __forceinline__ void rebuild_adjoint(int k, const Mom& m, double lf, double lg,
-                                                Mom* lm) {
-  const double cu = CX[k] * m.u + CY[k] * m.v;
-  const double p = 3.0 * cu + 4.5 * cu * cu - 1.5 * (m.u * m.u + m.v * m.v);
-  const double dpu = 3.0 * CX[k] + 9.0 * cu * CX[k] - 3.0 * m.u;
-  const double dpv = 3.0 * CY[k] + 9.0 * cu * CY[k] - 3.0 * m.v;
-  const double w = kW[k], cf = coef_of(k);
-  double hx, hy, hxy;
-  h_coef(k, &hx, &hy, &hxy);
-  lm->rho += lf * w * (1.0 + p);
-  lm->th += lg * w * (1.0 + p);
-  const double tt = kTcrit + m.th;
+template <typename A>
+__device__ __forceinline__ void rebuild_adjoint(int k, const Mom<A>& m, A lf, A lg, Mom<A>* lm) {
+  const A cu = A(CX[k]) * m.u + A(CY[k]) * m.v;
+  const A p = A(3) * cu + A(4.5) * cu * cu - A(1.5) * (m.u * m.u + m.v * m.v);
+  const A dpu = A(3 * CX[k]) + A(9) * cu * A(CX[k]) - A(3) * m.u;
+  const A dpv = A(3 * CY[k]) + A(9) * cu * A(CY[k]) - A(3) * m.v;
+  const A w = A(kW[k]), cf = A(coef_of(k));
+  A hx, hy, hxy;
+  h_coef<A>(k, &hx, &hy, &hxy);
+  lm->rho += lf * w * (A(1) + p);
+  lm->th += lg * w * (A(1) + p);
+  const A tt = A(kTcrit) + m.th;
   lm->u += (lf * m.rho + lg * tt) * w * dpu;
   lm->v += (lf * m.rho + lg * tt) * w * dpv;
   lm->pf[0] += lf * cf * hx;
@@ -216,16 +263,17 @@ __device__ __forceinline__ void rebuild_adjoint(int k, const Mom& m, double lf, 
 }
 
 // d ghost_gdev / dT for direction k at velocity (u, v)
-__device__ __forceinline__ double ghost_dT(int k, double u, double v) {
-  return kW[k] * (1.0 + eq_poly(k, u, v));
+template <typename A>
+__device__ __forceinline__ A ghost_dT(int k, A u, A v) {
+  return A(kW[k]) * (A(1) + eq_poly_t<A>(k, u, v));
 }
 
 // Pulled sums and taps of step n at node (i, j), window (ox, oy), from the
 // state before the step, with the boundary folding of the forward kernels
 // (mom_edge_nodes).
-template <typename R, bool EQF>
+template <typename R, bool EQF, typename A>
 __device__ __forceinline__ void pull_step(const AdjArgs& a, const Win<R, EQF>& W, int i, int j,
-                                          int ox, int oy, PullSums* S, double tap[9]) {
+                                          int ox, int oy, PullSumsT<A>* S, A tap[9]) {
   const int nx = a.gm.nx, ny = a.gm.ny;
   const int64_t P = a.gm.pitch;
   pull_zero(*S);
@@ -234,33 +282,36 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, const Win<R, EQF>& W
     const int si = i - CX[k], sj = j - CY[k];
     const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
     const int wx = ox - CX[k], wy = oy + (sjc - j);
-    double fk = 0.0, gk;
+    A fk = A(0), gk;
     if (si >= 1 && si <= nx) {
-      rebuild<R, EQF>(k, W.a[wy][wx], W.b[wy][wx], W.c[wy][wx], &fk, &gk);
+      rebuild<R, EQF, A, typename WinT<R>::A2>(k, W.a[wy][wx], W.b[wy][wx], W.c[wy][wx], &fk,
+                                               &gk);
     } else {
       const int64_t h = (int64_t)sjc * P + si;
-      gk = ghost_gdev(k, si == 0 ? a.c.tin : east_T(a.c, a.sprev.te, sjc), a.uh[h], a.vh[h]);
+      gk = ghost_gdev<A>(k, (A)(si == 0 ? a.c.tin : east_T(a.c, a.sprev.te, sjc)),
+                         (A)a.uh[h], (A)a.vh[h]);
     }
     if (si < 1 || si > nx || sj < 1 || sj > ny) {
       if (k == 6 && si == nx + 1 && sj == 0)
-        fk = a.c.frozen6;
+        fk = (A)a.c.frozen6;
       else if (k == 7 && si == nx + 1 && sj == ny + 1)
-        fk = a.c.frozen7;
+        fk = (A)a.c.frozen7;
       else
-        fk = rebuild_f<R, EQF>(OPP[k], W.a[oy][ox], W.b[oy][ox], W.c[oy][ox]);
+        fk = rebuild_f<R, EQF, A, typename WinT<R>::A2>(OPP[k], W.a[oy][ox], W.b[oy][ox],
+                                                        W.c[oy][ox]);
     }
     pull_add(*S, k, fk, gk);
   }
   const int yn = oy + (j < ny ? 1 : 0), ys = oy - (j > 1 ? 1 : 0);
-  tap[0] = W.phi[oy][ox];
-  tap[1] = W.phi[oy][ox + 1];
-  tap[2] = W.phi[yn][ox];
-  tap[3] = W.phi[oy][ox - 1];
-  tap[4] = W.phi[ys][ox];
-  tap[5] = W.phi[yn][ox + 1];
-  tap[6] = W.phi[yn][ox - 1];
-  tap[7] = W.phi[ys][ox - 1];
-  tap[8] = W.phi[ys][ox + 1];
+  tap[0] = (A)W.phi[oy][ox];
+  tap[1] = (A)W.phi[oy][ox + 1];
+  tap[2] = (A)W.phi[yn][ox];
+  tap[3] = (A)W.phi[oy][ox - 1];
+  tap[4] = (A)W.phi[ys][ox];
+  tap[5] = (A)W.phi[yn][ox + 1];
+  tap[6] = (A)W.phi[yn][ox - 1];
+  tap[7] = (A)W.phi[ys][ox - 1];
+  tap[8] = (A)W.phi[ys][ox + 1];
 }
 
 // Reverse of core_forward (bgk_mom.h) and of the state it stores.  lo: the
@@ -269,70 +320,71 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, const Win<R, EQF>& W
 // averaged T of moments.f90 seeds half of its adjoint here).  Returns the
 // adjoints of the pulled sums in lS, of the tap sums in lkp, lkm and of the
 // centre tap (direct terms) in lpc; adds parameter adjoints to g[3].
-__device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S,
-                                              const double tap[9], const Core& o, const Mom& lo,
-                                              double lphi_o, double ldT2_in, PullSums* lS,
-                                              double* lkp, double* lkm, double* lpc,
-                                              double g[3]) {
-  double lrho = lo.rho, lu = lo.u, lv = lo.v, lphi = lphi_o;
-  double ldT2 = ldT2_in;
+template <typename A>
+__device__ __forceinline__ void core_backward(const Consts& c, const PullSumsT<A>& S,
+                                              const A tap[9], const CoreT<A>& o, const Mom<A>& lo,
+                                              A lphi_o, A ldT2_in, PullSumsT<A>* lS, A* lkp,
+                                              A* lkm, A* lpc, double g[3]) {
+  A lrho = lo.rho, lu = lo.u, lv = lo.v, lphi = lphi_o;
+  A ldT2 = ldT2_in;
   // Pg = (1 - omega_g) Qg, Pf = (1 - omega_f) Qf
-  const double omg1 = 1.0 - o.omega_g, omf1 = 1.0 - c.omega_f;
-  const double lqgxx = omg1 * lo.pg[0], lqgyy = omg1 * lo.pg[1], lqgxy = omg1 * lo.pg[2];
-  const double lomg = -(o.qgxx * lo.pg[0] + o.qgyy * lo.pg[1] + o.qgxy * lo.pg[2]);
-  const double lqfxx = omf1 * lo.pf[0], lqfyy = omf1 * lo.pf[1], lqfxy = omf1 * lo.pf[2];
+  const A omg1 = A(1) - o.omega_g, omf1 = A(1) - (A)c.omega_f;
+  const A lqgxx = omg1 * lo.pg[0], lqgyy = omg1 * lo.pg[1], lqgxy = omg1 * lo.pg[2];
+  const A lomg = -(o.qgxx * lo.pg[0] + o.qgyy * lo.pg[1] + o.qgxy * lo.pg[2]);
+  const A lqfxx = omf1 * lo.pf[0], lqfyy = omf1 * lo.pf[1], lqfxy = omf1 * lo.pf[2];
   // Qg = S_g - cs2 dT2 - T u u
-  ldT2 += -kCs2 * (lqgxx + lqgyy);
-  double lT = -(o.u * o.u * lqgxx + o.v * o.v * lqgyy + o.u * o.v * lqgxy);
-  lu += -2.0 * o.T * o.u * lqgxx - o.T * o.v * lqgxy;
-  lv += -2.0 * o.T * o.v * lqgyy - o.T * o.u * lqgxy;
+  ldT2 += -A(kCs2) * (lqgxx + lqgyy);
+  A lT = -(o.u * o.u * lqgxx + o.v * o.v * lqgyy + o.u * o.v * lqgxy);
+  lu += -A(2) * o.T * o.u * lqgxx - o.T * o.v * lqgxy;
+  lv += -A(2) * o.T * o.v * lqgyy - o.T * o.u * lqgxy;
   // Qf = S_f - cs2 rho - rho u u
-  lrho += -(kCs2 + o.u * o.u) * lqfxx - (kCs2 + o.v * o.v) * lqfyy - o.u * o.v * lqfxy;
-  lu += -2.0 * o.rho * o.u * lqfxx - o.rho * o.v * lqfxy;
-  lv += -2.0 * o.rho * o.v * lqfyy - o.rho * o.u * lqfxy;
+  lrho += -(A(kCs2) + o.u * o.u) * lqfxx - (A(kCs2) + o.v * o.v) * lqfyy - o.u * o.v * lqfxy;
+  lu += -A(2) * o.rho * o.u * lqfxx - o.rho * o.v * lqfxy;
+  lv += -A(2) * o.rho * o.v * lqfyy - o.rho * o.u * lqfxy;
   // omega_g(phi, k_ratio) = 1 / (1/2 + alpha/cs2), alpha = alpha_l (fl +
   // k_ratio (1 - fl)): the k_ratio derivative is taken at k_ratio = 1 too,
   // where the forward uses the constant omega_g (the same value)
   {
-    const double fl = 0.5 * (1.0 + o.phi);
-    const double domg_dalpha = -o.omega_g * o.omega_g * 3.0;  // 1/cs2
-    lphi += lomg * domg_dalpha * c.alpha_l * (1.0 - c.k_ratio) * 0.5;
-    g[1] += lomg * domg_dalpha * c.alpha_l * (1.0 - fl);
+    const A fl = A(0.5) * (A(1) + o.phi);
+    const A domg_dalpha = -o.omega_g * o.omega_g * A(3);  // 1/cs2
+    lphi += lomg * domg_dalpha * (A)c.alpha_l * (A(1) - (A)c.k_ratio) * A(0.5);
+    g[1] += (double)(lomg * domg_dalpha * (A)c.alpha_l * (A(1) - fl));
   }
   // u = u2 + fx s, v = v2 + fy s, s = irho / omega_f
-  const double s = o.irho * c.inv_omega_f;
-  const double fx = -(o.u2 * (1.0 - o.phi) * 0.5);
-  const double gb = c.grav * c.beta;
-  const double fy0 = o.rho * gb * (o.T - c.t_ref) * (1.0 + o.phi * 0.5);
-  const double fy = fy0 - o.v2 * (1.0 - o.phi) * 0.5;
-  double lu2 = lu, lv2 = lv;
-  const double lfx = s * lu, lfy = s * lv;
-  double lirho = (fx * lu + fy * lv) * c.inv_omega_f;
+  const A s = o.irho * (A)c.inv_omega_f;
+  const A fx = -(o.u2 * (A(1) - o.phi) * A(0.5));
+  const A gb = (A)(c.grav * c.beta);
+  const A fy0 = o.rho * gb * (o.T - (A)c.t_ref) * (A(1) + o.phi * A(0.5));
+  const A fy = fy0 - o.v2 * (A(1) - o.phi) * A(0.5);
+  A lu2 = lu, lv2 = lv;
+  const A lfx = s * lu, lfy = s * lv;
+  A lirho = (fx * lu + fy * lv) * (A)c.inv_omega_f;
   // fy = fy0 - v2 (1 - phi)/2, fx = -u2 (1 - phi)/2
-  lv2 += -(1.0 - o.phi) * 0.5 * lfy;
-  lphi += o.v2 * 0.5 * lfy;
-  lu2 += -(1.0 - o.phi) * 0.5 * lfx;
-  lphi += o.u2 * 0.5 * lfx;
+  lv2 += -(A(1) - o.phi) * A(0.5) * lfy;
+  lphi += o.v2 * A(0.5) * lfy;
+  lu2 += -(A(1) - o.phi) * A(0.5) * lfx;
+  lphi += o.u2 * A(0.5) * lfx;
   // fy0 = rho g beta (T - t_ref) (1 + phi/2)
-  lrho += gb * (o.T - c.t_ref) * (1.0 + o.phi * 0.5) * lfy;
-  lT += o.rho * gb * (1.0 + o.phi * 0.5) * lfy;
-  lphi += o.rho * gb * (o.T - c.t_ref) * 0.5 * lfy;
+  lrho += gb * (o.T - (A)c.t_ref) * (A(1) + o.phi * A(0.5)) * lfy;
+  lT += o.rho * gb * (A(1) + o.phi * A(0.5)) * lfy;
+  lphi += o.rho * gb * (o.T - (A)c.t_ref) * A(0.5) * lfy;
   // stored T - T_crit = dT; T = T_crit + dT; dT = dT2 - dt_st (phi - pc)/2
-  const double ldT = lT + lo.th;
+  const A ldT = lT + lo.th;
   ldT2 += ldT;
-  lphi += -c.dt_st * 0.5 * ldT;
-  double lpcv = c.dt_st * 0.5 * ldT;
-  g[0] += -(o.phi - o.pc) * 0.5 * ldT * (-c.dt_st * c.inv_stefan);  // dt_st = delta_T / St
+  lphi += -(A)c.dt_st * A(0.5) * ldT;
+  A lpcv = (A)c.dt_st * A(0.5) * ldT;
+  // dt_st = delta_T / St
+  g[0] += (double)(-(o.phi - o.pc) * A(0.5) * ldT) * (-c.dt_st * c.inv_stefan);
   // phi = pc + 0.5 kpr (1 - pc) - 0.5 kmr (1 + pc), kpr = kp/8 rp, kmr = km/8 rm
-  const double rp = 1.0 - o.rm;
-  const double kpr = o.kp_sum * 0.125 * rp, kmr = o.km_sum * 0.125 * o.rm;
-  const double lkpr = 0.5 * (1.0 - o.pc) * lphi, lkmr = -0.5 * (1.0 + o.pc) * lphi;
-  lpcv += lphi * (1.0 - 0.5 * kpr - 0.5 * kmr);
-  *lkp = rp * 0.125 * lkpr;
-  *lkm = o.rm * 0.125 * lkmr;
-  const double lrm = o.km_sum * 0.125 * lkmr - o.kp_sum * 0.125 * lkpr;
+  const A rp = A(1) - o.rm;
+  const A kpr = o.kp_sum * A(0.125) * rp, kmr = o.km_sum * A(0.125) * o.rm;
+  const A lkpr = A(0.5) * (A(1) - o.pc) * lphi, lkmr = -A(0.5) * (A(1) + o.pc) * lphi;
+  lpcv += lphi * (A(1) - A(0.5) * kpr - A(0.5) * kmr);
+  *lkp = rp * A(0.125) * lkpr;
+  *lkm = o.rm * A(0.125) * lkmr;
+  const A lrm = o.km_sum * A(0.125) * lkmr - o.kp_sum * A(0.125) * lkpr;
   // rm = 1/(1 + exp((dT2 + crit_shift) * 2/T_width))
-  ldT2 += -o.rm * rp * c.two_over_tw * lrm;
+  ldT2 += -o.rm * rp * (A)c.two_over_tw * lrm;
   *lpc = lpcv;
   // u2 = jx irho, v2 = jy irho, irho = 1/rho
   lS->jx = o.irho * lu2;
@@ -352,6 +404,10 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSums& S
 template <typename R, bool EQF>
 __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjArgs a) {
   constexpr int BY = tile_rows<R, EQF>(), WY = BY + 2;
+  // arithmetic in the storage precision: FP64 for FP64 moments, FP32 for
+  // FP32 moments (the Jacobians evaluated in FP32: the kernel is bound by
+  // its arithmetic, which FP32 issues at twice the FP64 rate)
+  using A = R;
   const int nx = a.gm.nx, ny = a.gm.ny;
   const int64_t P = a.gm.pitch;
   const int i0 = blockIdx.x * BX + 1, j0 = blockIdx.y * BY + 1;
@@ -372,20 +428,36 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
       if (a.has_next)
 #pragma unroll
         for (int sl = 0; sl < S::N; ++sl) W.lam[sl][ey][ex] = plane_at<R>(a.next, S::plane[sl], x);
-      W.a[ey][ex] = a.sprev.a[x];
+      W.a[ey][ex] = WinT<R>::a2(a.sprev.a[x]);
       W.b[ey][ex] = b_at<R, EQF>(a.sprev.b, x);
       W.c[ey][ex] = c_at<R, EQF>(a.sprev.c, x);
-      W.phi[ey][ex] = a.sprev.phi[x];
+      W.phi[ey][ex] = (typename WinT<R>::P)a.sprev.phi[x];
     }
   }
   __syncthreads();
   double g[3] = {0.0, 0.0, 0.0};
   if (i <= nx && j <= ny) {
     const int64_t q = (int64_t)j * P + i;
-    const Mom m = load_mom<R, EQF>(a.sn, q);
-    Mom lm = {};
-    double lphi = 0.0;
-    double lTrep = 0.0;
+    // this node's state after step n: recomputed from the window (the same
+    // core_forward the forward kernels run), or the state itself for the
+    // initial-state launch
+    Mom<A> m;
+    A ph;
+    PullSumsT<A> S;
+    A tap[9];
+    CoreT<A> o;
+    if (a.core) {
+      pull_step<R, EQF, A>(a, W, i, j, ox, oy, &S, tap);
+      o = core_forward_t<A>(a.c, S, tap);
+      m = stored_mom<R, EQF, A>(a.c, o);
+      ph = o.phi;
+    } else {
+      m = win_mom<R, EQF, A, typename WinT<R>::A2>(W.a[oy][ox], W.b[oy][ox], W.c[oy][ox]);
+      ph = (A)W.phi[oy][ox];
+    }
+    Mom<A> lm = {};
+    A lphi = A(0);
+    A lTrep = A(0);
     if (a.has_next) {
       using S = LSlots<EQF>;
       // ---- reverse of step n+1's pulls from this node
@@ -393,7 +465,7 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
       for (int k = 0; k < 9; ++k) {
         const int xi = i + CX[k], xj = j + CY[k];
         if (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny)
-          rebuild_adjoint(k, m, lam_f(W, k, ox + CX[k], oy + CY[k]),
+          rebuild_adjoint<A>(k, m, lam_f(W, k, ox + CX[k], oy + CY[k]),
                           lam_g(W, k, ox + CX[k], oy + CY[k]), &lm);
         // this node's own pull of direction k from a ghost: bounce-back of
         // its opposite population (not for the two frozen corner ghosts)
@@ -401,7 +473,7 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
         if (si < 1 || si > nx || sj < 1 || sj > ny) {
           const bool frozen = (k == 6 && si == nx + 1 && sj == 0) ||
                               (k == 7 && si == nx + 1 && sj == ny + 1);
-          if (!frozen) rebuild_adjoint(OPP[k], m, lam_f(W, k, ox, oy), 0.0, &lm);
+          if (!frozen) rebuild_adjoint<A>(OPP[k], m, lam_f(W, k, ox, oy), A(0), &lm);
         }
       }
       // ghost rows of g: row 0 is a copy of row 1, row Ny+1 of row Ny
@@ -410,7 +482,7 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
         for (int k = 0; k < 9; ++k)
           if (CY[k] > 0) {
             const int xi = i + CX[k];
-            if (xi >= 1 && xi <= nx) rebuild_adjoint(k, m, 0.0, lam_g(W, k, ox + CX[k], oy), &lm);
+            if (xi >= 1 && xi <= nx) rebuild_adjoint<A>(k, m, A(0), lam_g(W, k, ox + CX[k], oy), &lm);
           }
       }
       if (j == ny) {
@@ -418,7 +490,7 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
         for (int k = 0; k < 9; ++k)
           if (CY[k] < 0) {
             const int xi = i + CX[k];
-            if (xi >= 1 && xi <= nx) rebuild_adjoint(k, m, 0.0, lam_g(W, k, ox + CX[k], oy), &lm);
+            if (xi >= 1 && xi <= nx) rebuild_adjoint<A>(k, m, A(0), lam_g(W, k, ox + CX[k], oy), &lm);
           }
       }
       // east ghost: equilibrium at this node's averaged T of step n (not
@@ -433,15 +505,14 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
             const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
             if (sjc != j) continue;
             const int64_t h = (int64_t)j * P + nx + 1;
-            lTrep += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(W, k, ox, oy + (xj - j));
+            lTrep += ghost_dT<A>(k, (A)a.uh[h], (A)a.vh[h]) * lam_g(W, k, ox, oy + (xj - j));
           }
         }
       }
       // ---- phi of step n as a tap of step n+1 (C, E, N, W, S, NE, NW, SW, SE)
       constexpr int DX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
       constexpr int DY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-      const double ph = a.sn.phi[q];
-      const double aa = 0.5 * (1.0 + ph), bb = 0.5 * (1.0 - ph);
+      const A aa = A(0.5) * (A(1) + ph), bb = A(0.5) * (A(1) - ph);
 #pragma unroll
       for (int t = 0; t < 9; ++t) {
         // nodes x whose tap t reads this node: x = y - d, and in rows 1 / Ny
@@ -458,7 +529,7 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
           }
         }
       }
-      lphi += plane_at<R>(a.next, L_PC, q);
+      lphi += (A)plane_at<R>(a.next, L_PC, q);
     }
     if (a.core) {
       // ---- loss seed at a probe node (averaged T after step n); column 0
@@ -466,18 +537,14 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
       // rows skip the per-node index load
       if (a.seeds && a.probe_idx[(int64_t)j * P] >= 0) {
         const int p = a.probe_idx[q];
-        if (p >= 0) lTrep += a.seeds[p];
+        if (p >= 0) lTrep += (A)a.seeds[p];
       }
       // averaged T = T_crit + (dT + dT2)/2: half on the stored dT, half on dT2
-      lm.th += 0.5 * lTrep;
+      lm.th += A(0.5) * lTrep;
       // ---- reverse of step n at this node
-      PullSums S;
-      double tap[9];
-      pull_step<R, EQF>(a, W, i, j, ox, oy, &S, tap);
-      const Core o = core_forward(a.c, S, tap);
-      PullSums lS;
-      double lkp, lkm, lpc;
-      core_backward(a.c, S, tap, o, lm, lphi, 0.5 * lTrep, &lS, &lkp, &lkm, &lpc, g);
+      PullSumsT<A> lS;
+      A lkp, lkm, lpc;
+      core_backward<A>(a.c, S, tap, o, lm, lphi, A(0.5) * lTrep, &lS, &lkp, &lkm, &lpc, g);
       const AdjPlanes& v = a.cur;
       plane_set<R>(v, L_RHO, q, lS.rho);
       plane_set<R>(v, L_JX, q, lS.jx);
@@ -505,12 +572,12 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
           const int sj = j - CY[k];
           const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
           const int64_t h = (int64_t)sjc * P;
-          g[2] += ghost_dT(k, a.uh[h], a.vh[h]) * lam_g(W, k, ox, oy);
+          g[2] += (double)(ghost_dT<A>(k, (A)a.uh[h], (A)a.vh[h]) * lam_g(W, k, ox, oy));
         }
     }
     if (!a.core && i <= a.interf) {
       // initial state: T = t_hot in the liquid strip (initialization.f90:15-26)
-      g[2] += lm.th;
+      g[2] += (double)lm.th;
     }
   }
   // ---- parameter adjoints: workgroup sum into this workgroup's own slot
@@ -578,11 +645,16 @@ struct AdjCtx {
   int seg1 = 0;   // outer segment (two levels), 0: one level
   int nckpt = 0;  // checkpoints of the forward run (every seg1, or seg, steps)
   int nckpt2 = 0; // checkpoints of one outer segment (two levels)
+  int nsnap = 0;  // binomial schedule: snapshot slots (state 0's included); 0: levels
   void* pool = nullptr;
   size_t bytes = 0;
   std::vector<StatePtrs> slot;   // states 1..seg of an inner segment (slot[0] spare)
   std::vector<StatePtrs> ckpt;   // checkpoints of the forward run
   std::vector<StatePtrs> ckpt2;  // checkpoints of the outer segment being reversed
+  std::vector<StatePtrs> snap;   // binomial: snapshot slots
+  StatePtrs work[2] = {};        // binomial: states being advanced
+  std::vector<int> chain;        // binomial: steps stored by the forward run (state
+                                 // chain[k] in snap[k + 1], state 0 in snap[0])
   AdjPlanes lam[2];
   int* probe_idx = nullptr;
   int* probe_ij = nullptr;
@@ -592,8 +664,110 @@ struct AdjCtx {
   double* partial = nullptr;
   double* out3 = nullptr;
   int nblk = 0;
+  int64_t recomputed = 0;     // forward steps the last backward pass ran
   bool forward_done = false;
 };
+
+namespace {
+
+// ---- binomial checkpointing (Griewank's "revolve" schedule, written from
+// the published method: A. Griewank, A. Walther, ACM TOMS 26 (2000) 19-45).
+// Reversing l steps from a held state with s free snapshot slots costs, at
+// best, T(l, s) = r l - C(S + r, S + 1) forward steps, S = s + 1 and r the
+// least integer with C(S + r, S) >= l (T(l, 0) = l (l - 1)/2: every state
+// advanced to from the held one).  An optimal schedule advances to a split
+// m, stores that state, reverses [m, l) with s - 1 slots, frees the slot and
+// reverses [0, m) with s slots; m minimises m + T(l - m, s - 1) + T(m, s).
+
+int64_t rev_cost(int64_t l, int64_t s) {
+  if (l <= 1) return 0;
+  const int64_t S = s + 1;
+  int64_t r = 0, beta = 1;  // beta = C(S + r, S)
+  while (beta < l) {
+    ++r;
+    beta = beta * (S + r) / r;
+  }
+  return r * l - beta * r / (S + 1);  // C(S + r, S + 1) = C(S + r, S) r / (S + 1)
+}
+
+int64_t rev_split(int64_t l, int64_t s) {
+  int64_t best = 1, cost = -1;
+  for (int64_t m = 1; m < l; ++m) {
+    const int64_t c = m + rev_cost(l - m, s - 1) + rev_cost(m, s);
+    if (cost < 0 || c < cost) {
+      cost = c;
+      best = m;
+    }
+  }
+  return best;
+}
+
+// Steps the forward run stores (the splits of the schedule's rightmost
+// branch, which the recorded forward reaches anyway).
+std::vector<int> binomial_chain(int nsteps, int nsnap) {
+  std::vector<int> chain;
+  int a = 0, freeslots = nsnap - 1;
+  while (nsteps - a >= 2 && freeslots >= 1) {
+    a += (int)rev_split(nsteps - a, freeslots);
+    chain.push_back(a);
+    --freeslots;
+  }
+  return chain;
+}
+
+// The operations of reversing steps a+1..b (step k maps state k-1 to k)
+// from state a held in snapshot slot sa, with the snapshot slots `freev`
+// free: `advance(src_state_slot_or_-1, from, to, dst)` and `adjoint(step,
+// state_slot)` callbacks; work slots are -1 - w.  Returns false when a
+// callback fails.
+template <typename Advance, typename Adjoint>
+bool reverse_steps(int a, int b, int sa, std::vector<int>& freev, Advance& advance,
+                   Adjoint& adjoint) {
+  const int l = b - a;
+  if (l == 1) return adjoint(b, sa);
+  if (freev.empty()) {
+    for (int k = b; k > a; --k) {
+      int st = sa;
+      if (k - 1 > a) {
+        st = -1 - ((k - 1 - a - 1) & 1);  // the work slot the last step lands in
+        if (!advance(sa, a, k - 1, st)) return false;
+      }
+      if (!adjoint(k, st)) return false;
+    }
+    return true;
+  }
+  const int m = a + (int)rev_split(l, (int64_t)freev.size());
+  const int sm = freev.back();
+  freev.pop_back();
+  if (!advance(sa, a, m, sm)) return false;
+  if (!reverse_steps(m, b, sm, freev, advance, adjoint)) return false;
+  freev.push_back(sm);
+  return reverse_steps(a, m, sa, freev, advance, adjoint);
+}
+
+}  // namespace
+
+// Forward steps the backward pass of a binomial schedule runs (the whole
+// schedule's advances minus those of the stored chain, which the recorded
+// forward run did).
+int64_t binomial_recompute(int nsteps, int nsnap) {
+  const std::vector<int> chain = binomial_chain(nsteps, nsnap);
+  int64_t n = 0;
+  auto advance = [&](int, int from, int to, int) {
+    n += to - from;
+    return true;
+  };
+  auto adjoint = [&](int, int) { return true; };
+  int right = nsteps;
+  for (int k = (int)chain.size(); k >= 0; --k) {
+    const int a = k == 0 ? 0 : chain[(size_t)k - 1];
+    std::vector<int> freev;
+    for (int f = nsnap - 1; f > k; --f) freev.push_back(f);
+    reverse_steps(a, right, k, freev, advance, adjoint);
+    right = a;
+  }
+  return n;
+}
 
 namespace {
 
@@ -648,15 +822,10 @@ void carve_state(Bump& b, const bgk_handle* h, StatePtrs* s) {
   s->te = b.take<double>((size_t)h->ny + 2);
 }
 
-size_t carve_ctx(bgk_handle* h, AdjCtx* a, char* base) {
-  Bump b{base, 0};
+// device bytes of everything but the stored states
+size_t carve_fixed(bgk_handle* h, AdjCtx* a, Bump& b) {
   const size_t n = (size_t)h->plane_elems;
-  a->slot.assign((size_t)a->seg + 1, StatePtrs{});
-  a->ckpt.assign((size_t)a->nckpt, StatePtrs{});
-  a->ckpt2.assign((size_t)a->nckpt2, StatePtrs{});
-  for (auto& s : a->slot) carve_state(b, h, &s);
-  for (auto& s : a->ckpt) carve_state(b, h, &s);
-  for (auto& s : a->ckpt2) carve_state(b, h, &s);
+  const size_t before = b.off;
   for (int l = 0; l < 2; ++l)
     for (int v = 0; v < 13; ++v) {
       const bool zero = h->mom_eqf && (v == L_FXX || v == L_FYY || v == L_FXY);
@@ -669,7 +838,39 @@ size_t carve_ctx(bgk_handle* h, AdjCtx* a, char* base) {
   for (int m = 0; m < 4; ++m) a->macro[m] = b.plane<double>(n);
   a->partial = b.take<double>(3 * (size_t)a->nblk);
   a->out3 = b.take<double>(3);
+  return b.off - before;
+}
+
+size_t carve_ctx(bgk_handle* h, AdjCtx* a, char* base) {
+  Bump b{base, 0};
+  if (a->nsnap > 0) {
+    a->slot.clear();
+    a->ckpt.clear();
+    a->ckpt2.clear();
+    a->snap.assign((size_t)a->nsnap, StatePtrs{});
+    for (auto& s : a->snap) carve_state(b, h, &s);
+    for (auto& s : a->work) carve_state(b, h, &s);
+  } else {
+    a->slot.assign((size_t)a->seg + 1, StatePtrs{});
+    a->ckpt.assign((size_t)a->nckpt, StatePtrs{});
+    a->ckpt2.assign((size_t)a->nckpt2, StatePtrs{});
+    for (auto& s : a->slot) carve_state(b, h, &s);
+    for (auto& s : a->ckpt) carve_state(b, h, &s);
+    for (auto& s : a->ckpt2) carve_state(b, h, &s);
+  }
+  carve_fixed(h, a, b);
   return b.off + 256;
+}
+
+// device bytes of one stored state and of the rest of a context
+size_t state_bytes(bgk_handle* h) {
+  Bump b{nullptr, 0};
+  StatePtrs s;
+  carve_state(b, h, &s);
+  carve_state(b, h, &s);
+  Bump b1{nullptr, 0};
+  carve_state(b1, h, &s);
+  return b.off - b1.off;
 }
 
 hipError_t copy_state(bgk_handle* h, const StatePtrs& dst, const double2* a, const void* bb,
@@ -688,10 +889,11 @@ hipError_t copy_state(bgk_handle* h, const StatePtrs& dst, const double2* a, con
   return e;
 }
 
-hipError_t launch_adjoint(bgk_handle* h, AdjCtx* x, const StatePtrs& sn, const StatePtrs& sprev,
-                          int lam_next, int has_next, int core, const double* seeds) {
+// adjoint of the step from state `sprev` (core = 1), or the initial state's
+// share (core = 0, sprev = the initial state)
+hipError_t launch_adjoint(bgk_handle* h, AdjCtx* x, const StatePtrs& sprev, int lam_next,
+                          int has_next, int core, const double* seeds) {
   AdjArgs a;
-  a.sn = sn;
   a.sprev = sprev;
   a.uh = h->u;
   a.vh = h->v;
@@ -745,19 +947,38 @@ void adj_free(bgk_handle* h) {
   h->adj = nullptr;
 }
 
-hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg, int np,
+int64_t adj_state_bytes(bgk_handle* h) { return (int64_t)state_bytes(h); }
+
+int64_t adj_bytes(bgk_handle* h, int nsteps, int np, int nsnap) {
+  AdjCtx x;
+  x.nsteps = nsteps;
+  x.np = np;
+  x.nsnap = nsnap;
+  const int by = tile_rows_of(h);
+  x.nblk = (int)(((h->nx + BX - 1) / BX) * ((h->ny + by - 1) / by));
+  return (int64_t)carve_ctx(h, &x, nullptr);
+}
+
+// nsnap > 0: binomial schedule with nsnap snapshot slots; else the level
+// schedule (seg_outer > 0: two levels)
+hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg, int nsnap, int np,
                      const int32_t* pi, const int32_t* pj) {
   adj_free(h);
   AdjCtx* x = new (std::nothrow) AdjCtx();
   if (!x) return hipErrorOutOfMemory;
   x->nsteps = nsteps;
-  x->seg = seg;
-  x->seg1 = seg_outer;
   x->np = np;
-  const int lvl1 = seg_outer > 0 ? seg_outer : seg;
-  x->nckpt = (nsteps + lvl1 - 1) / lvl1;
-  // (the first checkpoint of an outer segment is the outer checkpoint)
-  x->nckpt2 = seg_outer > 0 ? (std::min(seg_outer, nsteps) + seg - 1) / seg - 1 : 0;
+  x->nsnap = nsnap;
+  if (nsnap > 0) {
+    x->chain = binomial_chain(nsteps, nsnap);
+  } else {
+    x->seg = seg;
+    x->seg1 = seg_outer;
+    const int lvl1 = seg_outer > 0 ? seg_outer : seg;
+    x->nckpt = (nsteps + lvl1 - 1) / lvl1;
+    // (the first checkpoint of an outer segment is the outer checkpoint)
+    x->nckpt2 = seg_outer > 0 ? (std::min(seg_outer, nsteps) + seg - 1) / seg - 1 : 0;
+  }
   const int by = tile_rows_of(h);
   x->nblk = (int)(((h->nx + BX - 1) / BX) * ((h->ny + by - 1) / by));
   x->bytes = carve_ctx(h, x, nullptr);
@@ -812,13 +1033,25 @@ hipError_t adj_forward(bgk_handle* h, double* probe_T_host) {
   if (e == hipSuccess) e = launch_teast_prologue(h);
   if (e != hipSuccess) return e;
   int te = 0;  // teast[te] holds the current column
+  size_t next_chain = 0;
   for (int n = 0; n < x->nsteps; ++n) {
     const int a = h->cur, b = 1 - h->cur;
     StatePtrs in{h->ma[a], h->mb[a], h->mc[a], h->phi[h->pc], h->teast[te]};
     StatePtrs out{h->ma[b], h->mb[b], h->mc[b], h->phi[1 - h->pc], h->teast[1 - te]};
-    const int lvl1 = x->seg1 > 0 ? x->seg1 : x->seg;
-    if (n % lvl1 == 0) {
-      e = copy_state(h, x->ckpt[n / lvl1], in.a, in.b, in.c, in.phi, in.te);
+    const StatePtrs* keep = nullptr;
+    if (x->nsnap > 0) {
+      if (n == 0) {
+        keep = &x->snap[0];
+      } else if (next_chain < x->chain.size() && x->chain[next_chain] == n) {
+        keep = &x->snap[next_chain + 1];
+        ++next_chain;
+      }
+    } else {
+      const int lvl1 = x->seg1 > 0 ? x->seg1 : x->seg;
+      if (n % lvl1 == 0) keep = &x->ckpt[(size_t)(n / lvl1)];
+    }
+    if (keep) {
+      e = copy_state(h, *keep, in.a, in.b, in.c, in.phi, in.te);
       if (e != hipSuccess) return e;
     }
     e = fwd_step(h, in, out, x->probe_T + (size_t)n * x->np, x->probe_idx, n == x->nsteps - 1,
@@ -840,12 +1073,16 @@ hipError_t adj_forward(bgk_handle* h, double* probe_T_host) {
 void adj_info(const bgk_handle* h, int64_t* bytes, int32_t* n_states) {
   const AdjCtx* x = h->adj;
   *bytes = (int64_t)x->bytes;
-  *n_states = (int32_t)(x->slot.size() + x->ckpt.size() + x->ckpt2.size());
+  *n_states = x->nsnap > 0 ? (int32_t)(x->snap.size() + 2)
+                           : (int32_t)(x->slot.size() + x->ckpt.size() + x->ckpt2.size());
 }
+
+int64_t adj_recomputed(const bgk_handle* h) { return h->adj ? h->adj->recomputed : 0; }
 
 hipError_t adj_backward(bgk_handle* h, const double* seeds_host, double grad[3]) {
   AdjCtx* x = h->adj;
   hipError_t e = hipSuccess;
+  x->recomputed = 0;
   if (x->np > 0)
     e = hipMemcpyAsync(x->seeds, seeds_host, (size_t)x->nsteps * x->np * sizeof(double),
                        hipMemcpyHostToDevice, h->stream);
@@ -853,45 +1090,74 @@ hipError_t adj_backward(bgk_handle* h, const double* seeds_host, double grad[3])
     e = hipMemsetAsync(x->partial, 0, 3 * (size_t)x->nblk * sizeof(double), h->stream);
   int ln = 0;  // lam[ln] holds the adjoint of the step after the current one
   bool has_next = false;
-  // steps n0..n1-1 from `start` (the state before step n0): recompute every
-  // state of the segment (state n0 is `start` itself), then reverse them
-  auto inner = [&](const StatePtrs& start, int n0, int n1) {
-    auto st = [&](int n) -> const StatePtrs& { return n == n0 ? start : x->slot[n - n0]; };
-    for (int n = n0; n < n1 && e == hipSuccess; ++n)
-      e = fwd_step(h, st(n), st(n + 1), nullptr, nullptr, false, x->macro);
-    for (int n = n1; n > n0 && e == hipSuccess; --n) {
-      const double* sd = x->np > 0 ? x->seeds + (size_t)(n - 1) * x->np : nullptr;
-      e = launch_adjoint(h, x, st(n), st(n - 1), ln, has_next ? 1 : 0, 1, sd);
-      ln = 1 - ln;
-      has_next = true;
-    }
+  // adjoint of step n (1-based) from the state before it
+  auto adjoint_step = [&](int n, const StatePtrs& before) {
+    const double* sd = x->np > 0 ? x->seeds + (size_t)(n - 1) * x->np : nullptr;
+    e = launch_adjoint(h, x, before, ln, has_next ? 1 : 0, 1, sd);
+    ln = 1 - ln;
+    has_next = true;
+    return e == hipSuccess;
   };
-  if (x->seg1 == 0) {
-    for (int s = x->nckpt - 1; s >= 0 && e == hipSuccess; --s)
-      inner(x->ckpt[s], s * x->seg, std::min((s + 1) * x->seg, x->nsteps));
-  } else {
-    for (int s1 = x->nckpt - 1; s1 >= 0 && e == hipSuccess; --s1) {
-      const int m0 = s1 * x->seg1, m1 = std::min(m0 + x->seg1, x->nsteps);
-      const int ns2 = (m1 - m0 + x->seg - 1) / x->seg;
-      // checkpoints of this outer segment: c2(k) = state m0 + k seg, c2(0)
-      // the outer checkpoint itself; the steps in between go through
-      // slot[0] and slot[1]
-      auto c2 = [&](int k) -> const StatePtrs& { return k == 0 ? x->ckpt[s1] : x->ckpt2[k - 1]; };
-      for (int k = 1; k < ns2 && e == hipSuccess; ++k) {
-        const StatePtrs* src = &c2(k - 1);
-        for (int i = 0; i < x->seg && e == hipSuccess; ++i) {
-          const StatePtrs* dst = (i == x->seg - 1) ? &x->ckpt2[k - 1] : &x->slot[i & 1];
-          e = fwd_step(h, *src, *dst, nullptr, nullptr, false, x->macro);
-          src = dst;
-        }
+  auto step = [&](const StatePtrs& in, const StatePtrs& out) {
+    ++x->recomputed;
+    e = fwd_step(h, in, out, nullptr, nullptr, false, x->macro);
+    return e == hipSuccess;
+  };
+  if (x->nsnap > 0) {
+    // binomial schedule: slots >= 0 are snapshots, -1 and -2 the work states
+    auto at = [&](int sl) -> const StatePtrs& { return sl >= 0 ? x->snap[(size_t)sl] : x->work[-1 - sl]; };
+    auto advance = [&](int src, int from, int to, int dst) {
+      for (int t = from; t < to; ++t) {
+        const StatePtrs& in = t == from ? at(src) : x->work[(t - from - 1) & 1];
+        const StatePtrs& out = t == to - 1 ? at(dst) : x->work[(t - from) & 1];
+        if (!step(in, out)) return false;
       }
-      for (int s2 = ns2 - 1; s2 >= 0 && e == hipSuccess; --s2)
-        inner(c2(s2), m0 + s2 * x->seg, std::min(m0 + (s2 + 1) * x->seg, m1));
+      return true;
+    };
+    auto adjoint = [&](int n, int sl) { return adjoint_step(n, at(sl)); };
+    int right = x->nsteps;
+    for (int k = (int)x->chain.size(); k >= 0 && e == hipSuccess; --k) {
+      const int a = k == 0 ? 0 : x->chain[(size_t)k - 1];
+      std::vector<int> freev;
+      for (int f = x->nsnap - 1; f > k; --f) freev.push_back(f);
+      reverse_steps(a, right, k, freev, advance, adjoint);
+      right = a;
+    }
+  } else {
+    // steps n0..n1-1 from `start` (the state before step n0): recompute every
+    // state of the segment (state n0 is `start` itself), then reverse them
+    auto inner = [&](const StatePtrs& start, int n0, int n1) {
+      auto st = [&](int n) -> const StatePtrs& { return n == n0 ? start : x->slot[n - n0]; };
+      for (int n = n0 + 1; n < n1 && e == hipSuccess; ++n) step(st(n - 1), st(n));
+      for (int n = n1; n > n0 && e == hipSuccess; --n) adjoint_step(n, st(n - 1));
+    };
+    if (x->seg1 == 0) {
+      for (int s = x->nckpt - 1; s >= 0 && e == hipSuccess; --s)
+        inner(x->ckpt[s], s * x->seg, std::min((s + 1) * x->seg, x->nsteps));
+    } else {
+      for (int s1 = x->nckpt - 1; s1 >= 0 && e == hipSuccess; --s1) {
+        const int m0 = s1 * x->seg1, m1 = std::min(m0 + x->seg1, x->nsteps);
+        const int ns2 = (m1 - m0 + x->seg - 1) / x->seg;
+        // checkpoints of this outer segment: c2(k) = state m0 + k seg, c2(0)
+        // the outer checkpoint itself; the steps in between go through
+        // slot[0] and slot[1]
+        auto c2 = [&](int k) -> const StatePtrs& { return k == 0 ? x->ckpt[s1] : x->ckpt2[k - 1]; };
+        for (int k = 1; k < ns2 && e == hipSuccess; ++k) {
+          const StatePtrs* src = &c2(k - 1);
+          for (int i = 0; i < x->seg && e == hipSuccess; ++i) {
+            const StatePtrs* dst = (i == x->seg - 1) ? &x->ckpt2[k - 1] : &x->slot[i & 1];
+            step(*src, *dst);
+            src = dst;
+          }
+        }
+        for (int s2 = ns2 - 1; s2 >= 0 && e == hipSuccess; --s2)
+          inner(c2(s2), m0 + s2 * x->seg, std::min(m0 + (s2 + 1) * x->seg, m1));
+      }
     }
   }
   // the initial state's share (hot-wall temperature of the liquid strip)
-  if (e == hipSuccess)
-    e = launch_adjoint(h, x, x->ckpt[0], x->ckpt[0], ln, has_next ? 1 : 0, 0, nullptr);
+  const StatePtrs& s0 = x->nsnap > 0 ? x->snap[0] : x->ckpt[0];
+  if (e == hipSuccess) e = launch_adjoint(h, x, s0, ln, has_next ? 1 : 0, 0, nullptr);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, h->stream, x->partial, x->nblk, x->out3);
     e = hipGetLastError();

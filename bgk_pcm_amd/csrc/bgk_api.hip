@@ -285,6 +285,21 @@ int copy_small_back(bgk_handle* h, double* out, int n) {
   return BGK_OK;
 }
 
+// Probes of the adjoint: interior nodes, each at most once (each node
+// carries one probe slot on the device: a repeated node would leave one of
+// the two records unwritten).
+int check_probes(bgk_handle* h, int32_t n_probes, const int32_t* pi, const int32_t* pj) {
+  for (int32_t p = 0; p < n_probes; ++p)
+    if (pi[p] < 1 || pi[p] > h->nx || pj[p] < 1 || pj[p] > h->ny)
+      return fail(BGK_E_INVALID, "probe %d at (%d,%d) is not an interior node", p, pi[p], pj[p]);
+  std::vector<int64_t> q((size_t)n_probes);
+  for (int32_t p = 0; p < n_probes; ++p) q[(size_t)p] = (int64_t)pj[p] * (h->nx + 2) + pi[p];
+  std::sort(q.begin(), q.end());
+  if (std::adjacent_find(q.begin(), q.end()) != q.end())
+    return fail(BGK_E_INVALID, "two probes share one node; give each node at most once");
+  return BGK_OK;
+}
+
 // BGK_E_NONFINITE when one of out[0..n) is NaN or infinite: a sum over the
 // interior is non-finite exactly when a summed field holds a NaN or an
 // infinity (or the sum overflows), so the host test of the n results is
@@ -677,23 +692,54 @@ int bgk_adj_setup_levels(bgk_handle* h, int32_t nsteps, int32_t seg_outer, int32
       n_probes < 0 || (n_probes > 0 && (!pi || !pj)))
     return fail(BGK_E_INVALID, "nsteps=%d seg_outer=%d seg_inner=%d n_probes=%d", nsteps,
                 seg_outer, seg_inner, n_probes);
-  for (int32_t p = 0; p < n_probes; ++p)
-    if (pi[p] < 1 || pi[p] > h->nx || pj[p] < 1 || pj[p] > h->ny)
-      return fail(BGK_E_INVALID, "probe %d at (%d,%d) is not an interior node", p, pi[p], pj[p]);
-  {
-    // each node carries one probe slot on the device: a repeated node would
-    // leave one of the two records unwritten
-    std::vector<int64_t> q((size_t)n_probes);
-    for (int32_t p = 0; p < n_probes; ++p) q[(size_t)p] = (int64_t)pj[p] * (h->nx + 2) + pi[p];
-    std::sort(q.begin(), q.end());
-    if (std::adjacent_find(q.begin(), q.end()) != q.end())
-      return fail(BGK_E_INVALID, "two probes share one node; give each node at most once");
-  }
-  hipError_t e = bgk::adj_setup(h, nsteps, seg_outer, seg_inner, n_probes, pi, pj);
+  if ((rc = check_probes(h, n_probes, pi, pj))) return rc;
+  hipError_t e = bgk::adj_setup(h, nsteps, seg_outer, seg_inner, 0, n_probes, pi, pj);
   if (e == hipErrorOutOfMemory)
     return fail(BGK_E_NOMEM, "adjoint checkpoints of %d steps (segments %d, %d) do not fit",
                 nsteps, seg_outer, seg_inner);
   if (e != hipSuccess) return hip_fail(e, "adjoint setup");
+  return BGK_OK;
+}
+
+int bgk_adj_setup_binomial(bgk_handle* h, int32_t nsteps, int32_t n_states, int32_t n_probes,
+                           const int32_t* pi, const int32_t* pj) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (h->prm.precision != BGK_FP64_MOM && h->prm.precision != BGK_FP32_MOM)
+    return fail(BGK_E_UNSUPPORTED, "the adjoint differentiates the moment step: create the "
+                                   "handle with precision BGK_FP64_MOM or BGK_FP32_MOM");
+  if (nsteps < 1 || n_states < 0 || n_probes < 0 || (n_probes > 0 && (!pi || !pj)))
+    return fail(BGK_E_INVALID, "nsteps=%d n_states=%d n_probes=%d", nsteps, n_states, n_probes);
+  if ((rc = check_probes(h, n_probes, pi, pj))) return rc;
+  int nsnap = n_states;
+  if (nsnap == 0) {
+    // as many snapshots as the free device memory holds, 2 GiB kept back
+    bgk::adj_free(h);
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    const int64_t margin = (int64_t)2 << 30;
+    const int64_t one = bgk::adj_bytes(h, nsteps, n_probes, 1);
+    const int64_t per = bgk::adj_bytes(h, nsteps, n_probes, 2) - one;
+    const int64_t avail = (int64_t)fr - margin - one;
+    if (avail < 0)
+      return fail(BGK_E_NOMEM, "the adjoint of a %d x %d lattice needs %lld bytes; %zu free",
+                  h->nx, h->ny, (long long)(one + margin), fr);
+    nsnap = (int)std::min<int64_t>(1 + avail / per, nsteps);
+  }
+  nsnap = std::min(nsnap, nsteps);
+  hipError_t e = bgk::adj_setup(h, nsteps, 0, 0, nsnap, n_probes, pi, pj);
+  if (e == hipErrorOutOfMemory)
+    return fail(BGK_E_NOMEM, "binomial checkpoints of %d steps with %d states do not fit",
+                nsteps, nsnap);
+  if (e != hipSuccess) return hip_fail(e, "adjoint setup");
+  return BGK_OK;
+}
+
+int bgk_adj_plan(int32_t nsteps, int32_t n_states, int64_t* recomputed_steps) {
+  if (nsteps < 1 || n_states < 1 || !recomputed_steps)
+    return fail(BGK_E_INVALID, "nsteps=%d n_states=%d (both >= 1), output non-null", nsteps,
+                n_states);
+  *recomputed_steps = bgk::binomial_recompute(nsteps, std::min(n_states, nsteps));
   return BGK_OK;
 }
 

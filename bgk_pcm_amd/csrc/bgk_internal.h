@@ -139,8 +139,16 @@ hipError_t field_sums_planes(const double* a, const double* b, const double* c3,
 int max_scratch_devices();
 
 // ---- adjoint (bgk_adjoint.hip), FP64 moment handles
-hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg_inner, int np,
+// nsnap > 0: binomial checkpointing with nsnap snapshot slots; else the
+// level schedule (seg_outer > 0: two levels)
+hipError_t adj_setup(bgk_handle* h, int nsteps, int seg_outer, int seg_inner, int nsnap, int np,
                      const int32_t* pi, const int32_t* pj);
+// device bytes of a binomial context with nsnap snapshot slots
+int64_t adj_bytes(bgk_handle* h, int nsteps, int np, int nsnap);
+// forward steps the backward pass of a binomial schedule recomputes (host)
+int64_t binomial_recompute(int nsteps, int nsnap);
+// forward steps the last backward pass ran
+int64_t adj_recomputed(const bgk_handle* h);
 void adj_info(const bgk_handle* h, int64_t* bytes, int32_t* n_states);
 hipError_t adj_forward(bgk_handle* h, double* probe_T_host);
 hipError_t adj_backward(bgk_handle* h, const double* seeds_host, double grad[3]);

@@ -106,20 +106,28 @@ __device__ __forceinline__ typename CType<R, EQF>::T c_stream(const void* c, int
     return nt_load4<R>(static_cast<const typename Vec<R>::V4*>(c) + q);
 }
 
+// The helpers below take the arithmetic type A: double for the forward
+// kernels (and the FP64-moment adjoint), float for the adjoint of the FP32
+// moment step, which evaluates its Jacobians in FP32.
+
 // P_k(u,v): the equilibrium polynomial of direction k minus its leading 1.
-__device__ __forceinline__ double eq_poly(int k, double u, double v) {
-  const double uv = 1.5 * (u * u + v * v);
+template <typename A>
+__device__ __forceinline__ A eq_poly_t(int k, A u, A v) {
+  const A uv = A(1.5) * (u * u + v * v);
   switch (k) {
     case 0: return -uv;
-    case 1: return 3.0 * u + 4.5 * (u * u) - uv;
-    case 2: return 3.0 * v + 4.5 * (v * v) - uv;
-    case 3: return -3.0 * u + 4.5 * (u * u) - uv;
-    case 4: return -3.0 * v + 4.5 * (v * v) - uv;
-    case 5: { const double s = u + v; return 3.0 * s + 4.5 * (s * s) - uv; }
-    case 6: { const double s = u - v; return -3.0 * s + 4.5 * (s * s) - uv; }
-    case 7: { const double s = u + v; return -3.0 * s + 4.5 * (s * s) - uv; }
-    default: { const double s = u - v; return 3.0 * s + 4.5 * (s * s) - uv; }
+    case 1: return A(3) * u + A(4.5) * (u * u) - uv;
+    case 2: return A(3) * v + A(4.5) * (v * v) - uv;
+    case 3: return -A(3) * u + A(4.5) * (u * u) - uv;
+    case 4: return -A(3) * v + A(4.5) * (v * v) - uv;
+    case 5: { const A s = u + v; return A(3) * s + A(4.5) * (s * s) - uv; }
+    case 6: { const A s = u - v; return -A(3) * s + A(4.5) * (s * s) - uv; }
+    case 7: { const A s = u + v; return -A(3) * s + A(4.5) * (s * s) - uv; }
+    default: { const A s = u - v; return A(3) * s + A(4.5) * (s * s) - uv; }
   }
+}
+__device__ __forceinline__ double eq_poly(int k, double u, double v) {
+  return eq_poly_t<double>(k, u, v);
 }
 
 // H_k : P for P = (pxx, pyy, pxy), H_k = c_k c_k - cs2 I, in the storage
@@ -140,54 +148,59 @@ __device__ __forceinline__ R h_k(int k, R pxx, R pyy, R pxy) {
 }
 
 // Population k of a node from its moments: f_k and gd_k = g_k - w_k*T_crit.
-template <typename R, bool EQF>
-__device__ __forceinline__ void rebuild(int k, const double2 a, const typename Vec<R>::V4 b,
-                                        const typename CType<R, EQF>::T cc, double* fk,
-                                        double* gdk) {
-  const double p = eq_poly(k, (double)b.x, (double)b.y);
+template <typename R, bool EQF, typename A = double, typename A2 = double2>
+__device__ __forceinline__ void rebuild(int k, const A2 a, const typename Vec<R>::V4 b,
+                                        const typename CType<R, EQF>::T cc, A* fk, A* gdk) {
+  const A p = eq_poly_t<A>(k, (A)b.x, (A)b.y);
+  const A ax = (A)a.x, ay = (A)a.y;
   if constexpr (EQF) {
-    const double hg = (double)h_k<R>(k, b.z, b.w, cc);
-    *fk = kW[k] * a.x * (1.0 + p);
-    *gdk = kW[k] * (a.y + (kTcrit + a.y) * p) + coef_of(k) * hg;
+    const A hg = (A)h_k<R>(k, b.z, b.w, cc);
+    *fk = A(kW[k]) * ax * (A(1) + p);
+    *gdk = A(kW[k]) * (ay + (A(kTcrit) + ay) * p) + A(coef_of(k)) * hg;
   } else {
-    const double hf = (double)h_k<R>(k, b.z, b.w, cc.x);
-    const double hg = (double)h_k<R>(k, cc.y, cc.z, cc.w);
-    *fk = kW[k] * a.x * (1.0 + p) + coef_of(k) * hf;
-    *gdk = kW[k] * (a.y + (kTcrit + a.y) * p) + coef_of(k) * hg;
+    const A hf = (A)h_k<R>(k, b.z, b.w, cc.x);
+    const A hg = (A)h_k<R>(k, cc.y, cc.z, cc.w);
+    *fk = A(kW[k]) * ax * (A(1) + p) + A(coef_of(k)) * hf;
+    *gdk = A(kW[k]) * (ay + (A(kTcrit) + ay) * p) + A(coef_of(k)) * hg;
   }
 }
 
-template <typename R, bool EQF>
-__device__ __forceinline__ double rebuild_f(int k, const double2 a, const typename Vec<R>::V4 b,
-                                            const typename CType<R, EQF>::T cc) {
-  const double p = eq_poly(k, (double)b.x, (double)b.y);
+template <typename R, bool EQF, typename A = double, typename A2 = double2>
+__device__ __forceinline__ A rebuild_f(int k, const A2 a, const typename Vec<R>::V4 b,
+                                       const typename CType<R, EQF>::T cc) {
+  const A p = eq_poly_t<A>(k, (A)b.x, (A)b.y);
   if constexpr (EQF) {
-    return kW[k] * a.x * (1.0 + p);
+    return A(kW[k]) * (A)a.x * (A(1) + p);
   } else {
-    const double hf = (double)h_k<R>(k, b.z, b.w, cc.x);
-    return kW[k] * a.x * (1.0 + p) + coef_of(k) * hf;
+    const A hf = (A)h_k<R>(k, b.z, b.w, cc.x);
+    return A(kW[k]) * (A)a.x * (A(1) + p) + A(coef_of(k)) * hf;
   }
 }
 
 // Equilibrium deviation of a boundary ghost, g_k - w_k*T_crit.
-__device__ __forceinline__ double ghost_gdev(int k, double T, double u, double v) {
-  return kW[k] * (T - kTcrit) + kW[k] * T * eq_poly(k, u, v);
+template <typename A = double>
+__device__ __forceinline__ A ghost_gdev(int k, A T, A u, A v) {
+  return A(kW[k]) * (T - A(kTcrit)) + A(kW[k]) * T * eq_poly_t<A>(k, u, v);
 }
 
 // Moments of the pulled populations of a node: zeroth, first and second
 // moments of f, and of gd = g - w*T_crit (its first moment is not needed).
-struct PullSums {
-  double rho, jx, jy, fxx, fyy, fxy;
-  double dT, gxx, gyy, gxy;
+template <typename A>
+struct PullSumsT {
+  A rho, jx, jy, fxx, fyy, fxy;
+  A dT, gxx, gyy, gxy;
 };
+using PullSums = PullSumsT<double>;
 
-__device__ __forceinline__ void pull_zero(PullSums& S) {
-  S.rho = S.jx = S.jy = S.fxx = S.fyy = S.fxy = 0.0;
-  S.dT = S.gxx = S.gyy = S.gxy = 0.0;
+template <typename A>
+__device__ __forceinline__ void pull_zero(PullSumsT<A>& S) {
+  S.rho = S.jx = S.jy = S.fxx = S.fyy = S.fxy = A(0);
+  S.dT = S.gxx = S.gyy = S.gxy = A(0);
 }
 
 // Add population k (f_k, gd_k) to the sums.
-__device__ __forceinline__ void pull_add(PullSums& S, int k, double fk, double gk) {
+template <typename A>
+__device__ __forceinline__ void pull_add(PullSumsT<A>& S, int k, A fk, A gk) {
   S.rho += fk;
   S.dT += gk;
   if (CX[k] != 0) {
@@ -229,15 +242,17 @@ __device__ __forceinline__ void pull_add(PullSums& S, int k, double fk, double g
 // Off when k_ratio == 1: solid/liquid conductivity ratio through
 // a phase-dependent thermal relaxation, alpha(phi) = alpha_l (fl + k_ratio
 // (1 - fl)), fl = (1 + phi)/2 the liquid fraction, tau_g = 1/2 + alpha/cs2.
-struct Core {
-  double rho, irho, u2, v2, dT2;     // momentsPre
-  double kp_sum, km_sum, rm, phi, pc;  // phase field
-  double dT, T;                        // after latent heat
-  double u, v;                         // after forces
-  double omega_g;                      // thermal relaxation rate used
-  double qfxx, qfyy, qfxy;             // Pi_f^neq
-  double qgxx, qgyy, qgxy;             // Pi_g^neq
+template <typename A>
+struct CoreT {
+  A rho, irho, u2, v2, dT2;     // momentsPre
+  A kp_sum, km_sum, rm, phi, pc;  // phase field
+  A dT, T;                        // after latent heat
+  A u, v;                         // after forces
+  A omega_g;                      // thermal relaxation rate used
+  A qfxx, qfyy, qfxy;             // Pi_f^neq
+  A qgxx, qgyy, qgxy;             // Pi_g^neq
 };
+using Core = CoreT<double>;
 
 // 1/d as the hardware estimate refined by two Newton steps (r <- r + r (1 -
 // d r), each doubling the correct bits): within an ulp or two of the IEEE
@@ -251,17 +266,24 @@ __device__ __forceinline__ double rcp_nr(double d) {
   r = fma(r, fma(-d, r, 1.0), r);
   return r;
 }
-
-__device__ __forceinline__ double omega_g_of(const Consts& c, double phi) {
-  if (c.k_ratio == 1.0) return c.omega_g;
-  const double fl = 0.5 * (1.0 + phi);
-  const double alpha = c.alpha_l * (fl + c.k_ratio * (1.0 - fl));
-  return rcp_nr(0.5 + alpha * 3.0);  // 1/(1/2 + alpha/cs2)
+// FP32: the hardware estimate and one Newton step
+__device__ __forceinline__ float rcp_nr(float d) {
+  float r = __builtin_amdgcn_rcpf(d);
+  return fmaf(r, fmaf(-d, r, 1.0f), r);
 }
 
-__device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
-                                             const double tap[9]) {
-  Core o;
+template <typename A>
+__device__ __forceinline__ A omega_g_of(const Consts& c, A phi) {
+  if (c.k_ratio == 1.0) return (A)c.omega_g;
+  const A fl = A(0.5) * (A(1) + phi);
+  const A alpha = (A)c.alpha_l * (fl + (A)c.k_ratio * (A(1) - fl));
+  return rcp_nr(A(0.5) + alpha * A(3));  // 1/(1/2 + alpha/cs2)
+}
+
+template <typename A>
+__device__ __forceinline__ CoreT<A> core_forward_t(const Consts& c, const PullSumsT<A>& S,
+                                                   const A tap[9]) {
+  CoreT<A> o;
   o.rho = S.rho;
   o.dT2 = S.dT;
   o.irho = rcp_nr(o.rho);
@@ -269,14 +291,14 @@ __device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
   o.v2 = S.jy * o.irho;
 
   // sum_m ((1 +/- t_m)/2)^2 = (9 +/- 2 sum t + sum t^2) / 4
-  double st = tap[0], st2 = tap[0] * tap[0];
+  A st = tap[0], st2 = tap[0] * tap[0];
 #pragma unroll
   for (int m = 1; m < 9; ++m) {
     st = st + tap[m];
     st2 = fma(tap[m], tap[m], st2);
   }
-  const double kp = 0.25 * ((9.0 + st2) + 2.0 * st);
-  const double km = 0.25 * ((9.0 + st2) - 2.0 * st);
+  const A kp = A(0.25) * ((A(9) + st2) + A(2) * st);
+  const A km = A(0.25) * ((A(9) + st2) - A(2) * st);
   o.kp_sum = kp;
   o.km_sum = km;
   o.pc = tap[0];
@@ -284,31 +306,36 @@ __device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
   // and km = 0 (all taps +1) or pc = -1: phi = pc whatever the rates, which
   // are then not evaluated (rm = 1/2 keeps every product below finite; the
   // adjoint multiplies its rate terms by the same exact zeros).
-  const bool react = !((kp == 0.0 || o.pc == 1.0) && (km == 0.0 || o.pc == -1.0));
-  o.rm = 0.5;
-  if (react) o.rm = 1.0 / (1.0 + exp((o.dT2 + c.crit_shift) * c.two_over_tw));
-  const double rp = 1.0 - o.rm;
-  const double kpr = (kp * 0.125) * rp;
-  const double kmr = (km * 0.125) * o.rm;
-  o.phi = o.pc + (0.5 * kpr * (1.0 - o.pc) - 0.5 * kmr * (1.0 + o.pc));
+  const bool react = !((kp == A(0) || o.pc == A(1)) && (km == A(0) || o.pc == A(-1)));
+  o.rm = A(0.5);
+  if (react) o.rm = A(1) / (A(1) + exp((o.dT2 + (A)c.crit_shift) * (A)c.two_over_tw));
+  const A rp = A(1) - o.rm;
+  const A kpr = (kp * A(0.125)) * rp;
+  const A kmr = (km * A(0.125)) * o.rm;
+  o.phi = o.pc + (A(0.5) * kpr * (A(1) - o.pc) - A(0.5) * kmr * (A(1) + o.pc));
 
-  o.dT = o.dT2 - (c.dt_st * (o.phi - o.pc) * 0.5);
-  o.T = kTcrit + o.dT;
-  double fy = o.rho * c.grav * c.beta * (o.T - c.t_ref) * (1.0 + o.phi * 0.5);
-  const double fx = -(o.u2 * (1.0 - o.phi) * 0.5);
-  fy = fy - o.v2 * (1.0 - o.phi) * 0.5;
-  const double s = o.irho * c.inv_omega_f;
+  o.dT = o.dT2 - ((A)c.dt_st * (o.phi - o.pc) * A(0.5));
+  o.T = A(kTcrit) + o.dT;
+  A fy = o.rho * (A)c.grav * (A)c.beta * (o.T - (A)c.t_ref) * (A(1) + o.phi * A(0.5));
+  const A fx = -(o.u2 * (A(1) - o.phi) * A(0.5));
+  fy = fy - o.v2 * (A(1) - o.phi) * A(0.5);
+  const A s = o.irho * (A)c.inv_omega_f;
   o.u = o.u2 + fx * s;
   o.v = o.v2 + fy * s;
 
-  o.omega_g = omega_g_of(c, o.phi);
-  o.qfxx = S.fxx - kCs2 * o.rho - o.rho * o.u * o.u;
-  o.qfyy = S.fyy - kCs2 * o.rho - o.rho * o.v * o.v;
+  o.omega_g = omega_g_of<A>(c, o.phi);
+  o.qfxx = S.fxx - A(kCs2) * o.rho - o.rho * o.u * o.u;
+  o.qfyy = S.fyy - A(kCs2) * o.rho - o.rho * o.v * o.v;
   o.qfxy = S.fxy - o.rho * o.u * o.v;
-  o.qgxx = S.gxx - kCs2 * o.dT2 - o.T * o.u * o.u;
-  o.qgyy = S.gyy - kCs2 * o.dT2 - o.T * o.v * o.v;
+  o.qgxx = S.gxx - A(kCs2) * o.dT2 - o.T * o.u * o.u;
+  o.qgyy = S.gyy - A(kCs2) * o.dT2 - o.T * o.v * o.v;
   o.qgxy = S.gxy - o.T * o.u * o.v;
   return o;
+}
+
+__device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
+                                             const double tap[9]) {
+  return core_forward_t<double>(c, S, tap);
 }
 
 // Averaged moments of STEP_7/moments.f90:11-25 from the stored state: sum_k

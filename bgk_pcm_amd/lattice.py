@@ -102,6 +102,8 @@ SIGNATURES = {
     "bgk_adj_setup_levels": [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                              _I32P, _I32P],
     "bgk_adj_info": [_H, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)],
+    "bgk_adj_setup_binomial": [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _I32P, _I32P],
+    "bgk_adj_plan": [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)],
     "bgk_adj_forward": [_H, _D],
     "bgk_adj_backward": [_H, _D, _D],
 }
@@ -124,6 +126,13 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     if path is None:
         _lib = L
     return L
+
+
+def adj_plan(nsteps: int, n_states: int) -> int:
+    """Forward steps the backward pass of a binomial schedule recomputes."""
+    out = ctypes.c_int64()
+    _check(load_library().bgk_adj_plan(nsteps, n_states, ctypes.byref(out)))
+    return int(out.value)
 
 
 def _check(rc: int) -> None:
@@ -176,6 +185,17 @@ class Lattice:
         aj = np.ascontiguousarray(pj, dtype=np.int32)
         _check(self._lib.bgk_adj_setup_levels(self._h, nsteps, seg_outer, seg_len, len(pi),
                                               ai.ctypes.data_as(_I32P), aj.ctypes.data_as(_I32P)))
+
+    def adj_setup_binomial(self, nsteps: int, n_states: int, pi: Sequence[int],
+                           pj: Sequence[int]) -> None:
+        """Binomial checkpointing with n_states snapshot slots (0: as many as
+        the free device memory holds)."""
+        self._adj_n, self._adj_np = int(nsteps), len(pi)
+        ai = np.ascontiguousarray(pi, dtype=np.int32)
+        aj = np.ascontiguousarray(pj, dtype=np.int32)
+        _check(self._lib.bgk_adj_setup_binomial(self._h, nsteps, n_states, len(pi),
+                                                ai.ctypes.data_as(_I32P),
+                                                aj.ctypes.data_as(_I32P)))
 
     def adj_info(self) -> tuple[int, int]:
         """(device bytes of the adjoint's checkpoints and state, stored lattice states)."""

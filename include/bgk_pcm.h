@@ -293,6 +293,23 @@ int bgk_adj_setup(bgk_handle* h, int32_t nsteps, int32_t seg_len, int32_t n_prob
 int bgk_adj_setup_levels(bgk_handle* h, int32_t nsteps, int32_t seg_outer, int32_t seg_inner,
                          int32_t n_probes, const int32_t* pi, const int32_t* pj);
 int bgk_adj_info(bgk_handle* h, int64_t* device_bytes, int32_t* n_states);
+
+/* Binomial checkpointing (the "revolve" schedule of Griewank and Walther,
+ * ACM TOMS 26 (2000)) in place of checkpoint levels: n_states snapshot slots
+ * (the initial state's included; 0: as many as the free device memory holds,
+ * 2 GiB kept back) plus two working states.  The forward run stores the
+ * states the schedule's first branch needs; the backward pass recomputes the
+ * minimal number of forward steps for that memory (bgk_adj_plan), about
+ * 1.2 per step for 5000 steps with 88 snapshots where two checkpoint levels
+ * recompute 2.  bgk_adj_forward and bgk_adj_backward as above; the gradient
+ * equals the level schedules' bit for bit (the same states are recomputed). */
+int bgk_adj_setup_binomial(bgk_handle* h, int32_t nsteps, int32_t n_states, int32_t n_probes,
+                           const int32_t* pi, const int32_t* pj);
+
+/* Forward steps the backward pass of a binomial schedule of nsteps steps
+ * with n_states snapshot slots recomputes (host arithmetic only, no handle
+ * or device needed): to choose the memory a run spends. */
+int bgk_adj_plan(int32_t nsteps, int32_t n_states, int64_t* recomputed_steps);
 int bgk_adj_forward(bgk_handle* h, double* probe_T);
 int bgk_adj_backward(bgk_handle* h, const double* seeds, double grad[3]);
 

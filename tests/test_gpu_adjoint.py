@@ -299,3 +299,30 @@ def test_adjoint_with_critical_and_reference_temperatures_matches_central_differ
         fd = (loss(rec_at(tp)) - loss(rec_at(tm))) / (2 * h)
         print(f"component {c}: adjoint {g[c]:.10e} central difference {fd:.10e}")
         assert abs(fd - g[c]) <= 1e-4 * abs(g[c]), (c, fd, g[c])
+
+
+@pytest.mark.parametrize("n_states", [1, 2, 3, 5, 8, 0])
+def test_binomial_schedule_gives_the_level_schedules_gradient(n_states):
+    """Binomial checkpointing (bgk_adj_setup_binomial) recomputes the same
+    states as the one-level schedule: the same gradient bit for bit, with 1
+    snapshot (every state recomputed from the initial one) up to as many as
+    memory holds (0: capped at the number of steps, every state stored)."""
+    p = Params(nx=30, ny=24, length=30.0, grav=2e-3, rounding=ROUND_CLEAN, tau_f=0.8,
+               stefan_number=0.1, precision=FP64_MOM)
+    nsteps = 75
+    pi, pj = seeded_probes(p.nx, p.ny, 6, seed=5)
+    theta0 = (0.1, 1.4, 283.5)
+    target = gpu_record(p, (0.12, 1.2, 283.2), pi, pj, nsteps, 10)
+    grads = []
+    for mode in ("levels", "binomial"):
+        print(f"adjoint {mode}, {nsteps} steps, n_states {n_states}", flush=True)
+        with Lattice(p.replace(stefan_number=theta0[0]), initialize=False, t_hot=theta0[2],
+                     k_ratio=theta0[1]) as lat:
+            lat.initialization()
+            if mode == "levels":
+                lat.adj_setup(nsteps, 9, pi, pj)
+            else:
+                lat.adj_setup_binomial(nsteps, n_states, pi, pj)
+            rec = lat.adj_forward()
+            grads.append(lat.adj_backward(2.0 * (rec - target) / rec.size))
+    assert np.array_equal(grads[0], grads[1]), grads

@@ -16,9 +16,10 @@ sizes and lengths.
    max-norm, liquid fraction for phi) after 5000, 10000 and 20000 steps.
  - config 4, 512^2 adjoint: the gradient of the probe loss against central
    differences of the GPU forward (step 1e-6), within 1e-4.
- - config 5, 8192^2 FP32 forward + adjoint with two levels of checkpoints
-   fits in HBM and gives a finite gradient; at 1024^2 the FP32 adjoint is
-   within 1e-4 of the FP64 adjoint.
+ - config 5, 8192^2 FP32 forward + adjoint with binomial checkpointing in
+   the memory the device has free fits in HBM and gives a finite gradient;
+   at 1024^2 the FP32 adjoint is within 1e-4 of the FP64 adjoint (two
+   checkpoint levels there).
 """
 import hashlib
 import json
@@ -194,7 +195,7 @@ def test_config3_fp32_storage_within_1e5_at_every_mark(config3_fp64, precision):
 
 # ------------------------------------------------------------------ config 4
 
-def adjoint_problem(w, n_probes=16, seg_outer=0, seg=100):
+def adjoint_problem(w, n_probes=16, seg_outer=0, seg=100, binomial=False):
     base = w.params
     pi, pj = seeded_probes(base.nx, base.ny, n_probes, seed=0)
 
@@ -203,7 +204,10 @@ def adjoint_problem(w, n_probes=16, seg_outer=0, seg=100):
         lat = Lattice(base.replace(stefan_number=st), initialize=False,
                       t_hot=configs.t_hot_of(base, th), k_ratio=kr)
         lat.initialization()
-        lat.adj_setup(w.nsteps, seg, pi, pj, seg_outer=seg_outer)
+        if binomial:
+            lat.adj_setup_binomial(w.nsteps, 0, pi, pj)
+        else:
+            lat.adj_setup(w.nsteps, seg, pi, pj, seg_outer=seg_outer)
         rec = lat.adj_forward()
         if keep:
             return lat, rec
@@ -243,7 +247,7 @@ def test_config4_adjoint_matches_central_differences():
 def test_config5_fp32_adjoint_fits_and_matches_fp64_at_1024():
     from bgk_pcm_amd.lattice import Lattice as L
     w = configs.config5()
-    record = adjoint_problem(w, seg_outer=289, seg=17)
+    record = adjoint_problem(w, binomial=True)
     print("config 5, 8192^2: target trajectory (5000 steps)", flush=True)
     target = record(configs.ADJ_TARGET)
     lat, rec = record(configs.ADJ_THETA0, keep=True)
