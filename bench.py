@@ -172,7 +172,10 @@ def measure(p, stream, steps, warmup, kind, kernel, ext=None):
         lat.step(steps)
         lat.synchronize()
         wall = (time.perf_counter() - t0) * 1e3 / steps
-        kms, extra = per_launch(lat, stream)
+        # FP32 populations re-base their reference density every 500 steps:
+        # calls of 500 and 1000 steps hold one and two re-basings, so the
+        # difference is 500 launches plus one re-basing (its amortized cost)
+        kms, extra = per_launch(lat, stream, m=500 if kind == "fp32" else 200)
         lat.field_check()  # raises on a non-finite field
     nbytes = BYTES_PER_LUP[kind] * lups
     ach = nbytes / (kms * 1e-3) / 1e9

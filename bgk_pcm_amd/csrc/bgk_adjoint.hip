@@ -12,15 +12,18 @@
 // the node update depends on (PullSums of bgk_mom.h: rho, j, the second
 // moments of f; T and the second moments of g) and of the two tap sums and
 // the centre tap of the phase field: 13 values, 10 with omega_f = 1, stored
-// in the state's precision and computed in FP64.  The pull fp_k(x) =
+// and computed in the state's precision.  The pull fp_k(x) =
 // f_k(x - c_k) is reversed as a gather, so no atomics: node y collects,
 // for every direction k, the adjoint of the population k that node y + c_k
 // pulled from it, and turns it into the adjoint of its own stored moments
 // through the derivative of the population rebuild.  The node update is
 // reversed by hand (core_backward below), on values recomputed with the
-// same core_forward the forward kernels run.
+// same core_forward the forward kernels run: the adjoint of step n reads
+// only the state before step n.
 //
-// Checkpointing: bgk_adj_forward keeps the state every seg_len steps;
+// Checkpointing, binomial (bgk_adj_setup_binomial): the optimal schedule for
+// a number of state slots (see rev_cost below).  Levels (bgk_adj_setup,
+// bgk_adj_setup_levels): bgk_adj_forward keeps the state every seg_len steps;
 // bgk_adj_backward recomputes one segment at a time from its checkpoint,
 // keeping every state of the segment, then runs the adjoint kernels over
 // the segment backwards.  Memory: (nsteps/seg_len + seg_len + 1) states.
@@ -80,6 +83,7 @@ struct AdjArgs {
   Consts c;
   Geom gm;
   int interf;
+  int ntx, nty;  // tiles across and down
 };
 
 // The workgroup's window in LDS: a tile of BX x BY nodes and a one-node
@@ -89,17 +93,16 @@ struct AdjArgs {
 // neither stored nor read) and the state before step n (moments and phi).
 // Every read of a node's update lies in its 3 x 3 neighbourhood, so each
 // value comes from HBM once per tile instead of from L2 for every neighbour.
-// Tiles of 64 x 4 nodes (one workgroup of 4 waves) or 64 x 8 (8 waves),
-// whichever keeps more waves per SIMD.  The window holds 80 B per node with
-// FP32 moments and omega_f = 1 (32 KB for 64 x 4: five workgroups per CU,
-// 5 waves per SIMD once the registers fit in 102, which the launch bound
-// asks for: 96 VGPRs, 3 spilled), 104 B with the general FP32 layout (8
-// waves per 69 KB at 64 x 8: 4 per SIMD), 136 B and 168 B with FP64 moments
-// (64 x 4: 3 and 2 waves per SIMD).  Measured at 8192^2 (FP32, omega_f = 1)
-// per launch: 64 x 4 3.18 ms at 5 waves per SIMD, 4.33 ms at 3 (the planes
-// then held in FP64 in LDS); 64 x 8 3.66 ms (4 waves); 64 x 16 4.29 ms (one
-// workgroup per CU, so none computes while it fills its window).  The halo
-// re-reads (1.55x the tile's nodes at 64 x 4, 1.29x at 64 x 8) are L2 hits.
+// Tiles of 64 x 8 nodes (one workgroup of 8 waves) with FP32 moments, 64 x 4
+// (4 waves) with FP64 moments.  The window holds 68 B per node with FP32
+// moments and omega_f = 1 (45 KB at 64 x 8: three workgroups per CU, 6
+// waves per SIMD, 72 VGPRs), 92 B with the general FP32 layout (61 KB: 4
+// waves per SIMD), 136 B and 168 B with FP64 moments (64 x 4: 3 and 2 waves
+// per SIMD).  Measured at 8192^2 (FP32, omega_f = 1, XCD bands of 8-tile
+// groups) per launch: 64 x 8 2.05 ms, 64 x 12 (two workgroups of 12 waves)
+// 2.07 ms, 64 x 4 (six of 4 waves) 2.19 ms.  Per wave: 765 VALU
+// instructions, and HBM reads of 1.17x the algorithmic bytes (the halo
+// rows and columns are L2 hits, see tile_of).
 constexpr int BX = 64, WX = BX + 2;
 template <typename R, bool EQF>
 constexpr int tile_rows() {
@@ -230,36 +233,59 @@ __device__ __forceinline__ Mom<A> win_mom(const A2& a, const typename Vec<R>::V4
   return m;
 }
 
-// d(H_k : P)/dP for P = (xx, yy, xy)
+// Weighted moments sum_k w_k c_k^n a_k (n = 0, 1, 2) of a vector over the
+// nine directions.
 template <typename A>
-__device__ __forceinline__ void h_coef(int k, A* hx, A* hy, A* hxy) {
-  *hx = (CX[k] != 0 ? A(1) : A(0)) - A(kCs2);
-  *hy = (CY[k] != 0 ? A(1) : A(0)) - A(kCs2);
-  *hxy = A(2 * CX[k] * CY[k]);
+struct WMom {
+  A m0, mx, my, mxx, myy, mxy;
+};
+template <typename A>
+__device__ __forceinline__ WMom<A> wmoments(const A a[9]) {
+  const A s13 = a[1] + a[3], s24 = a[2] + a[4];
+  const A sd = (a[5] + a[6]) + (a[7] + a[8]);
+  WMom<A> m;
+  m.m0 = A(kW[0]) * a[0] + A(kW[1]) * (s13 + s24) + A(kW[5]) * sd;
+  m.mx = A(kW[1]) * (a[1] - a[3]) + A(kW[5]) * ((a[5] + a[8]) - (a[6] + a[7]));
+  m.my = A(kW[1]) * (a[2] - a[4]) + A(kW[5]) * ((a[5] + a[6]) - (a[7] + a[8]));
+  m.mxx = A(kW[1]) * s13 + A(kW[5]) * sd;
+  m.myy = A(kW[1]) * s24 + A(kW[5]) * sd;
+  m.mxy = A(kW[5]) * ((a[5] + a[7]) - (a[6] + a[8]));
+  return m;
 }
 
-// Accumulate into lm the adjoint of the node's moments from the adjoints lf,
-// lg of its populations f_k and gd_k (the transposed rebuild Jacobian).
+// Accumulate into lm the adjoint of the node's moments from the adjoints
+// lf[k], lg[k] of its nine populations f_k and gd_k (the transposed rebuild
+// Jacobian).  Population k is w_k rho (1 + p_k) + c_k H_k : Pf for f and
+// w_k (dT + T p_k) + c_k H_k : Pg for gd, with p_k = 3 cu + 9/2 cu^2 -
+// 3/2 |u|^2, cu = c_k . u, c_k = w_k / (2 cs^4) and H_k = c_k c_k - cs^2 I;
+// its derivatives are polynomials in c_k, so the sums over k reduce to the
+// weighted moments of lf and lg.
 template <typename A>
-__device__ __forceinline__ void rebuild_adjoint(int k, const Mom<A>& m, A lf, A lg, Mom<A>* lm) {
-  const A cu = A(CX[k]) * m.u + A(CY[k]) * m.v;
-  const A p = A(3) * cu + A(4.5) * cu * cu - A(1.5) * (m.u * m.u + m.v * m.v);
-  const A dpu = A(3 * CX[k]) + A(9) * cu * A(CX[k]) - A(3) * m.u;
-  const A dpv = A(3 * CY[k]) + A(9) * cu * A(CY[k]) - A(3) * m.v;
-  const A w = A(kW[k]), cf = A(coef_of(k));
-  A hx, hy, hxy;
-  h_coef<A>(k, &hx, &hy, &hxy);
-  lm->rho += lf * w * (A(1) + p);
-  lm->th += lg * w * (A(1) + p);
-  const A tt = A(kTcrit) + m.th;
-  lm->u += (lf * m.rho + lg * tt) * w * dpu;
-  lm->v += (lf * m.rho + lg * tt) * w * dpv;
-  lm->pf[0] += lf * cf * hx;
-  lm->pf[1] += lf * cf * hy;
-  lm->pf[2] += lf * cf * hxy;
-  lm->pg[0] += lg * cf * hx;
-  lm->pg[1] += lg * cf * hy;
-  lm->pg[2] += lg * cf * hxy;
+__device__ __forceinline__ void rebuild_adjoint(const Mom<A>& m, const A lf[9], const A lg[9],
+                                                Mom<A>* lm) {
+  const WMom<A> F = wmoments<A>(lf), G = wmoments<A>(lg);
+  const A u = m.u, v = m.v, uu = u * u, vv = v * v, uv2 = A(2) * u * v;
+  const A c0 = A(1) - A(1.5) * (uu + vv);
+  // sum_k w_k a_k (1 + p_k)
+  lm->rho += F.m0 * c0 + A(3) * (u * F.mx + v * F.my) +
+             A(4.5) * (uu * F.mxx + uv2 * F.mxy + vv * F.myy);
+  lm->th += G.m0 * c0 + A(3) * (u * G.mx + v * G.my) +
+            A(4.5) * (uu * G.mxx + uv2 * G.mxy + vv * G.myy);
+  // sum_k w_k L_k dp_k/du, L_k = rho lf_k + T lg_k (and dv)
+  const A rho = m.rho, tt = A(kTcrit) + m.th;
+  const A L0 = rho * F.m0 + tt * G.m0;
+  const A Lx = rho * F.mx + tt * G.mx, Ly = rho * F.my + tt * G.my;
+  const A Lxx = rho * F.mxx + tt * G.mxx, Lyy = rho * F.myy + tt * G.myy;
+  const A Lxy = rho * F.mxy + tt * G.mxy;
+  lm->u += A(3) * (Lx - u * L0) + A(9) * (u * Lxx + v * Lxy);
+  lm->v += A(3) * (Ly - v * L0) + A(9) * (u * Lxy + v * Lyy);
+  // sum_k c_k H_k a_k = (9/2) (Mxx - cs2 M0, Myy - cs2 M0, 2 Mxy)
+  lm->pf[0] += A(4.5) * (F.mxx - A(kCs2) * F.m0);
+  lm->pf[1] += A(4.5) * (F.myy - A(kCs2) * F.m0);
+  lm->pf[2] += A(9) * F.mxy;
+  lm->pg[0] += A(4.5) * (G.mxx - A(kCs2) * G.m0);
+  lm->pg[1] += A(4.5) * (G.myy - A(kCs2) * G.m0);
+  lm->pg[2] += A(9) * G.mxy;
 }
 
 // d ghost_gdev / dT for direction k at velocity (u, v)
@@ -270,8 +296,9 @@ __device__ __forceinline__ A ghost_dT(int k, A u, A v) {
 
 // Pulled sums and taps of step n at node (i, j), window (ox, oy), from the
 // state before the step, with the boundary folding of the forward kernels
-// (mom_edge_nodes).
-template <typename R, bool EQF, typename A>
+// (mom_edge_nodes).  EDGE = false: the node and its eight neighbours are
+// interior nodes (no boundary tests).
+template <typename R, bool EQF, typename A, bool EDGE>
 __device__ __forceinline__ void pull_step(const AdjArgs& a, const Win<R, EQF>& W, int i, int j,
                                           int ox, int oy, PullSumsT<A>* S, A tap[9]) {
   const int nx = a.gm.nx, ny = a.gm.ny;
@@ -280,10 +307,10 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, const Win<R, EQF>& W
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     const int si = i - CX[k], sj = j - CY[k];
-    const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
+    const int sjc = !EDGE ? sj : (sj < 1 ? 1 : (sj > ny ? ny : sj));
     const int wx = ox - CX[k], wy = oy + (sjc - j);
     A fk = A(0), gk;
-    if (si >= 1 && si <= nx) {
+    if (!EDGE || (si >= 1 && si <= nx)) {
       rebuild<R, EQF, A, typename WinT<R>::A2>(k, W.a[wy][wx], W.b[wy][wx], W.c[wy][wx], &fk,
                                                &gk);
     } else {
@@ -291,7 +318,7 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, const Win<R, EQF>& W
       gk = ghost_gdev<A>(k, (A)(si == 0 ? a.c.tin : east_T(a.c, a.sprev.te, sjc)),
                          (A)a.uh[h], (A)a.vh[h]);
     }
-    if (si < 1 || si > nx || sj < 1 || sj > ny) {
+    if (EDGE && (si < 1 || si > nx || sj < 1 || sj > ny)) {
       if (k == 6 && si == nx + 1 && sj == 0)
         fk = (A)a.c.frozen6;
       else if (k == 7 && si == nx + 1 && sj == ny + 1)
@@ -302,7 +329,7 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, const Win<R, EQF>& W
     }
     pull_add(*S, k, fk, gk);
   }
-  const int yn = oy + (j < ny ? 1 : 0), ys = oy - (j > 1 ? 1 : 0);
+  const int yn = oy + (!EDGE || j < ny ? 1 : 0), ys = oy - (!EDGE || j > 1 ? 1 : 0);
   tap[0] = (A)W.phi[oy][ox];
   tap[1] = (A)W.phi[oy][ox + 1];
   tap[2] = (A)W.phi[yn][ox];
@@ -324,7 +351,7 @@ template <typename A>
 __device__ __forceinline__ void core_backward(const Consts& c, const PullSumsT<A>& S,
                                               const A tap[9], const CoreT<A>& o, const Mom<A>& lo,
                                               A lphi_o, A ldT2_in, PullSumsT<A>* lS, A* lkp,
-                                              A* lkm, A* lpc, double g[3]) {
+                                              A* lkm, A* lpc, A g[3]) {
   A lrho = lo.rho, lu = lo.u, lv = lo.v, lphi = lphi_o;
   A ldT2 = ldT2_in;
   // Pg = (1 - omega_g) Qg, Pf = (1 - omega_f) Qf
@@ -348,7 +375,7 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSumsT<A
     const A fl = A(0.5) * (A(1) + o.phi);
     const A domg_dalpha = -o.omega_g * o.omega_g * A(3);  // 1/cs2
     lphi += lomg * domg_dalpha * (A)c.alpha_l * (A(1) - (A)c.k_ratio) * A(0.5);
-    g[1] += (double)(lomg * domg_dalpha * (A)c.alpha_l * (A(1) - fl));
+    g[1] += lomg * domg_dalpha * (A)c.alpha_l * (A(1) - fl);
   }
   // u = u2 + fx s, v = v2 + fy s, s = irho / omega_f
   const A s = o.irho * (A)c.inv_omega_f;
@@ -374,7 +401,7 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSumsT<A
   lphi += -(A)c.dt_st * A(0.5) * ldT;
   A lpcv = (A)c.dt_st * A(0.5) * ldT;
   // dt_st = delta_T / St
-  g[0] += (double)(-(o.phi - o.pc) * A(0.5) * ldT) * (-c.dt_st * c.inv_stefan);
+  g[0] += (-(o.phi - o.pc) * A(0.5) * ldT) * (A)(-c.dt_st * c.inv_stefan);
   // phi = pc + 0.5 kpr (1 - pc) - 0.5 kmr (1 + pc), kpr = kp/8 rp, kmr = km/8 rm
   const A rp = A(1) - o.rm;
   const A kpr = o.kp_sum * A(0.125) * rp, kmr = o.km_sum * A(0.125) * o.rm;
@@ -401,16 +428,218 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSumsT<A
   lS->gxy = lqgxy;
 }
 
-template <typename R, bool EQF>
-__global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjArgs a) {
-  constexpr int BY = tile_rows<R, EQF>(), WY = BY + 2;
-  // arithmetic in the storage precision: FP64 for FP64 moments, FP32 for
-  // FP32 moments (the Jacobians evaluated in FP32: the kernel is bound by
-  // its arithmetic, which FP32 issues at twice the FP64 rate)
+// The adjoint of step n at node (i, j), window (ox, oy); the parameter
+// adjoints are added to g.  EDGE = false for the tiles whose nodes and their
+// neighbours are all interior nodes: no boundary terms and no tests.
+// Arithmetic in the storage precision: FP64 for FP64 moments, FP32 for FP32
+// moments (the Jacobians evaluated in FP32: the kernel is bound by its
+// arithmetic).
+template <typename R, bool EQF, bool EDGE>
+__device__ __forceinline__ void adjoint_node(const AdjArgs& a, const Win<R, EQF>& W, int i, int j,
+                                             int ox, int oy, R g[3]) {
   using A = R;
   const int nx = a.gm.nx, ny = a.gm.ny;
   const int64_t P = a.gm.pitch;
-  const int i0 = blockIdx.x * BX + 1, j0 = blockIdx.y * BY + 1;
+  const int64_t q = (int64_t)j * P + i;
+  // this node's state after step n: recomputed from the window (the same
+  // core_forward the forward kernels run), or the state itself for the
+  // initial-state launch
+  Mom<A> m;
+  A ph;
+  PullSumsT<A> S;
+  A tap[9];
+  CoreT<A> o;
+  if (a.core) {
+    pull_step<R, EQF, A, EDGE>(a, W, i, j, ox, oy, &S, tap);
+    o = core_forward_t<A>(a.c, S, tap);
+    m = stored_mom<R, EQF, A>(a.c, o);
+    ph = o.phi;
+  } else {
+    m = win_mom<R, EQF, A, typename WinT<R>::A2>(W.a[oy][ox], W.b[oy][ox], W.c[oy][ox]);
+    ph = (A)W.phi[oy][ox];
+  }
+  Mom<A> lm = {};
+  A lphi = A(0);
+  A lTrep = A(0);
+  if (a.has_next) {
+    using S = LSlots<EQF>;
+    // ---- reverse of step n+1's pulls from this node: the adjoints of its
+    // nine populations, then through the rebuild to its moments
+    A lf[9], lg[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int xi = i + CX[k], xj = j + CY[k];
+      if (!EDGE || (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny)) {
+        lf[k] = lam_f(W, k, ox + CX[k], oy + CY[k]);
+        lg[k] = lam_g(W, k, ox + CX[k], oy + CY[k]);
+      } else {
+        lf[k] = lg[k] = A(0);
+      }
+    }
+    if constexpr (EDGE) {
+      // this node's own pull of direction k from a ghost: bounce-back of
+      // its opposite population (not for the two frozen corner ghosts)
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const int si = i - CX[k], sj = j - CY[k];
+        if (si < 1 || si > nx || sj < 1 || sj > ny) {
+          const bool frozen = (k == 6 && si == nx + 1 && sj == 0) ||
+                              (k == 7 && si == nx + 1 && sj == ny + 1);
+          if (!frozen) lf[OPP[k]] += lam_f(W, k, ox, oy);
+        }
+      }
+      // ghost rows of g: row 0 is a copy of row 1, row Ny+1 of row Ny
+      if (j == 1) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+          if (CY[k] > 0) {
+            const int xi = i + CX[k];
+            if (xi >= 1 && xi <= nx) lg[k] += lam_g(W, k, ox + CX[k], oy);
+          }
+      }
+      if (j == ny) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+          if (CY[k] < 0) {
+            const int xi = i + CX[k];
+            if (xi >= 1 && xi <= nx) lg[k] += lam_g(W, k, ox + CX[k], oy);
+          }
+      }
+    }
+    rebuild_adjoint<A>(m, lf, lg, &lm);
+    // east ghost: equilibrium at this node's averaged T of step n (not
+    // with the cold-wall extension: then a constant)
+    if (EDGE && i == nx && !a.c.east_cold) {
+      for (int xj = j - 1; xj <= j + 1; ++xj) {
+        if (xj < 1 || xj > ny) continue;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          if (CX[k] >= 0) continue;  // directions 3, 6, 7 pull from column Nx+1
+          const int sj = xj - CY[k];
+          const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
+          if (sjc != j) continue;
+          const int64_t h = (int64_t)j * P + nx + 1;
+          lTrep += ghost_dT<A>(k, (A)a.uh[h], (A)a.vh[h]) * lam_g(W, k, ox, oy + (xj - j));
+        }
+      }
+    }
+    // ---- phi of step n as a tap of step n+1 (C, E, N, W, S, NE, NW, SW, SE)
+    constexpr int DX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    constexpr int DY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    A skp = A(0), skm = A(0);
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      // nodes x whose tap t reads this node: x = y - d, and in rows 1 / Ny
+      // also those whose tap t reads the ghost row copied from this node
+      const int xi = i - DX[t], xj = j - DY[t];
+      if (!EDGE || (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny)) {
+        const int wx = ox - DX[t], wy = oy - DY[t];
+        skp += W.lam[S::KP][wy][wx];
+        skm += W.lam[S::KM][wy][wx];
+      }
+      if (EDGE && ((j == 1 && DY[t] < 0) || (j == ny && DY[t] > 0))) {
+        if (xi >= 1 && xi <= nx) {
+          const int wx = ox - DX[t];
+          skp += W.lam[S::KP][oy][wx];
+          skm += W.lam[S::KM][oy][wx];
+        }
+      }
+    }
+    lphi = A(0.5) * (A(1) + ph) * skp - A(0.5) * (A(1) - ph) * skm;
+    lphi += (A)plane_at<R>(a.next, L_PC, q);
+  }
+  if (a.core) {
+    // ---- loss seed at a probe node (averaged T after step n); column 0
+    // of the index plane flags the rows that hold probes, so the other
+    // rows skip the per-node index load
+    if (a.seeds && a.probe_idx[(int64_t)j * P] >= 0) {
+      const int p = a.probe_idx[q];
+      if (p >= 0) lTrep += (A)a.seeds[p];
+    }
+    // averaged T = T_crit + (dT + dT2)/2: half on the stored dT, half on dT2
+    lm.th += A(0.5) * lTrep;
+    // ---- reverse of step n at this node
+    PullSumsT<A> lS;
+    A lkp, lkm, lpc;
+    core_backward<A>(a.c, S, tap, o, lm, lphi, A(0.5) * lTrep, &lS, &lkp, &lkm, &lpc, g);
+    const AdjPlanes& v = a.cur;
+    plane_set<R>(v, L_RHO, q, lS.rho);
+    plane_set<R>(v, L_JX, q, lS.jx);
+    plane_set<R>(v, L_JY, q, lS.jy);
+    if constexpr (!EQF) {
+      plane_set<R>(v, L_FXX, q, lS.fxx);
+      plane_set<R>(v, L_FYY, q, lS.fyy);
+      plane_set<R>(v, L_FXY, q, lS.fxy);
+    }
+    plane_set<R>(v, L_DT, q, lS.dT);
+    plane_set<R>(v, L_GXX, q, lS.gxx);
+    plane_set<R>(v, L_GYY, q, lS.gyy);
+    plane_set<R>(v, L_GXY, q, lS.gxy);
+    plane_set<R>(v, L_KP, q, lkp);
+    plane_set<R>(v, L_KM, q, lkm);
+    plane_set<R>(v, L_PC, q, lpc);
+  }
+  // this node's pulls from the west ghost (column 1): equilibria at the
+  // hot-wall temperature (left for last: no register holds the sum while
+  // the rest of the node is reversed)
+  if (EDGE && a.has_next && i == 1) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (CX[k] == 1) {
+        const int sj = j - CY[k];
+        const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
+        const int64_t h = (int64_t)sjc * P;
+        g[2] += ghost_dT<A>(k, (A)a.uh[h], (A)a.vh[h]) * lam_g(W, k, ox, oy);
+      }
+  }
+  if (!a.core && i <= a.interf) {
+    // initial state: T = t_hot in the liquid strip (initialization.f90:15-26)
+    g[2] += lm.th;
+  }
+}
+
+// Tile order.  gfx950 dispatches workgroup b to XCD b % 8 and each XCD has
+// its own L2.  XCD x takes the x-th of eight horizontal bands of tile rows
+// and walks it in groups of kGroup tile columns, row by row within a group:
+// the cache lines of a window's halo (the 4- or 8-byte columns left and right
+// of the tile and the rows above and below it) were then fetched into the
+// same L2 by a tile a few places earlier on the same XCD, instead of by
+// another XCD.  Returns false for the padding workgroups of the shorter bands.
+constexpr int kGroup = 8;
+__device__ __forceinline__ bool tile_of(int ntx, int nty, int b, int* tx, int* ty) {
+  const int x = b % NXCD, s = b / NXCD;
+  const int r0 = (int)((int64_t)x * nty / NXCD), r1 = (int)((int64_t)(x + 1) * nty / NXCD);
+  const int rows = r1 - r0;
+  if (s >= rows * ntx) return false;
+  const int g = s / (kGroup * rows);
+  const int r = s - g * kGroup * rows;
+  const int gw = ntx - g * kGroup < kGroup ? ntx - g * kGroup : kGroup;
+  *ty = r0 + r / gw;
+  *tx = g * kGroup + r % gw;
+  return true;
+}
+inline int tile_grid(int ntx, int nty) { return NXCD * ((nty + NXCD - 1) / NXCD) * ntx; }
+
+// Waves per SIMD that the LDS of a CU (160 KiB) holds for k_adjoint: the
+// register budget asked of the compiler (more registers would not raise the
+// occupancy).
+template <typename R, bool EQF>
+constexpr int adj_waves() {
+  constexpr int waves_per_wg = BX * tile_rows<R, EQF>() / 64;
+  constexpr int lds = (int)sizeof(Win<R, EQF>) + 3 * 8 * waves_per_wg;
+  constexpr int w = (160 * 1024 / lds) * waves_per_wg / 4;
+  return w > 8 ? 8 : (w < 1 ? 1 : w);
+}
+
+template <typename R, bool EQF>
+__global__ __launch_bounds__((BX * tile_rows<R, EQF>()), (adj_waves<R, EQF>())) void k_adjoint(
+    AdjArgs a) {
+  constexpr int BY = tile_rows<R, EQF>(), WY = BY + 2;
+  const int nx = a.gm.nx, ny = a.gm.ny;
+  const int64_t P = a.gm.pitch;
+  int tx, ty;
+  if (!tile_of(a.ntx, a.nty, (int)blockIdx.x, &tx, &ty)) return;  // whole workgroup
+  const int i0 = tx * BX + 1, j0 = ty * BY + 1;
   const int ox = threadIdx.x + 1, oy = threadIdx.y + 1;  // this node in the window
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   // ---- the window: rows j0-1 .. j0+BY, columns i0-1 .. i0+BX, clamped to
@@ -435,150 +664,17 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
     }
   }
   __syncthreads();
-  double g[3] = {0.0, 0.0, 0.0};
+  // parameter adjoints of this thread's node, in the arithmetic type (FP32
+  // terms are summed in FP32 over the 64 nodes of a wave, in FP64 beyond)
+  R g[3] = {R(0), R(0), R(0)};
+  // tiles whose nodes all have four interior neighbours skip every
+  // boundary test (a workgroup-uniform branch)
+  const bool edge = !(i0 > 1 && i0 + BX <= nx && j0 > 1 && j0 + BY <= ny);
   if (i <= nx && j <= ny) {
-    const int64_t q = (int64_t)j * P + i;
-    // this node's state after step n: recomputed from the window (the same
-    // core_forward the forward kernels run), or the state itself for the
-    // initial-state launch
-    Mom<A> m;
-    A ph;
-    PullSumsT<A> S;
-    A tap[9];
-    CoreT<A> o;
-    if (a.core) {
-      pull_step<R, EQF, A>(a, W, i, j, ox, oy, &S, tap);
-      o = core_forward_t<A>(a.c, S, tap);
-      m = stored_mom<R, EQF, A>(a.c, o);
-      ph = o.phi;
-    } else {
-      m = win_mom<R, EQF, A, typename WinT<R>::A2>(W.a[oy][ox], W.b[oy][ox], W.c[oy][ox]);
-      ph = (A)W.phi[oy][ox];
-    }
-    Mom<A> lm = {};
-    A lphi = A(0);
-    A lTrep = A(0);
-    if (a.has_next) {
-      using S = LSlots<EQF>;
-      // ---- reverse of step n+1's pulls from this node
-#pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        const int xi = i + CX[k], xj = j + CY[k];
-        if (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny)
-          rebuild_adjoint<A>(k, m, lam_f(W, k, ox + CX[k], oy + CY[k]),
-                          lam_g(W, k, ox + CX[k], oy + CY[k]), &lm);
-        // this node's own pull of direction k from a ghost: bounce-back of
-        // its opposite population (not for the two frozen corner ghosts)
-        const int si = i - CX[k], sj = j - CY[k];
-        if (si < 1 || si > nx || sj < 1 || sj > ny) {
-          const bool frozen = (k == 6 && si == nx + 1 && sj == 0) ||
-                              (k == 7 && si == nx + 1 && sj == ny + 1);
-          if (!frozen) rebuild_adjoint<A>(OPP[k], m, lam_f(W, k, ox, oy), A(0), &lm);
-        }
-      }
-      // ghost rows of g: row 0 is a copy of row 1, row Ny+1 of row Ny
-      if (j == 1) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-          if (CY[k] > 0) {
-            const int xi = i + CX[k];
-            if (xi >= 1 && xi <= nx) rebuild_adjoint<A>(k, m, A(0), lam_g(W, k, ox + CX[k], oy), &lm);
-          }
-      }
-      if (j == ny) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-          if (CY[k] < 0) {
-            const int xi = i + CX[k];
-            if (xi >= 1 && xi <= nx) rebuild_adjoint<A>(k, m, A(0), lam_g(W, k, ox + CX[k], oy), &lm);
-          }
-      }
-      // east ghost: equilibrium at this node's averaged T of step n (not
-      // with the cold-wall extension: then a constant)
-      if (i == nx && !a.c.east_cold) {
-        for (int xj = j - 1; xj <= j + 1; ++xj) {
-          if (xj < 1 || xj > ny) continue;
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            if (CX[k] >= 0) continue;  // directions 3, 6, 7 pull from column Nx+1
-            const int sj = xj - CY[k];
-            const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
-            if (sjc != j) continue;
-            const int64_t h = (int64_t)j * P + nx + 1;
-            lTrep += ghost_dT<A>(k, (A)a.uh[h], (A)a.vh[h]) * lam_g(W, k, ox, oy + (xj - j));
-          }
-        }
-      }
-      // ---- phi of step n as a tap of step n+1 (C, E, N, W, S, NE, NW, SW, SE)
-      constexpr int DX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-      constexpr int DY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-      const A aa = A(0.5) * (A(1) + ph), bb = A(0.5) * (A(1) - ph);
-#pragma unroll
-      for (int t = 0; t < 9; ++t) {
-        // nodes x whose tap t reads this node: x = y - d, and in rows 1 / Ny
-        // also those whose tap t reads the ghost row copied from this node
-        const int xi = i - DX[t], xj = j - DY[t];
-        if (xi >= 1 && xi <= nx && xj >= 1 && xj <= ny) {
-          const int wx = ox - DX[t], wy = oy - DY[t];
-          lphi += aa * W.lam[S::KP][wy][wx] - bb * W.lam[S::KM][wy][wx];
-        }
-        if ((j == 1 && DY[t] < 0) || (j == ny && DY[t] > 0)) {
-          if (xi >= 1 && xi <= nx) {
-            const int wx = ox - DX[t];
-            lphi += aa * W.lam[S::KP][oy][wx] - bb * W.lam[S::KM][oy][wx];
-          }
-        }
-      }
-      lphi += (A)plane_at<R>(a.next, L_PC, q);
-    }
-    if (a.core) {
-      // ---- loss seed at a probe node (averaged T after step n); column 0
-      // of the index plane flags the rows that hold probes, so the other
-      // rows skip the per-node index load
-      if (a.seeds && a.probe_idx[(int64_t)j * P] >= 0) {
-        const int p = a.probe_idx[q];
-        if (p >= 0) lTrep += (A)a.seeds[p];
-      }
-      // averaged T = T_crit + (dT + dT2)/2: half on the stored dT, half on dT2
-      lm.th += A(0.5) * lTrep;
-      // ---- reverse of step n at this node
-      PullSumsT<A> lS;
-      A lkp, lkm, lpc;
-      core_backward<A>(a.c, S, tap, o, lm, lphi, A(0.5) * lTrep, &lS, &lkp, &lkm, &lpc, g);
-      const AdjPlanes& v = a.cur;
-      plane_set<R>(v, L_RHO, q, lS.rho);
-      plane_set<R>(v, L_JX, q, lS.jx);
-      plane_set<R>(v, L_JY, q, lS.jy);
-      if constexpr (!EQF) {
-        plane_set<R>(v, L_FXX, q, lS.fxx);
-        plane_set<R>(v, L_FYY, q, lS.fyy);
-        plane_set<R>(v, L_FXY, q, lS.fxy);
-      }
-      plane_set<R>(v, L_DT, q, lS.dT);
-      plane_set<R>(v, L_GXX, q, lS.gxx);
-      plane_set<R>(v, L_GYY, q, lS.gyy);
-      plane_set<R>(v, L_GXY, q, lS.gxy);
-      plane_set<R>(v, L_KP, q, lkp);
-      plane_set<R>(v, L_KM, q, lkm);
-      plane_set<R>(v, L_PC, q, lpc);
-    }
-    // this node's pulls from the west ghost (column 1): equilibria at the
-    // hot-wall temperature (left for last: no register holds the sum while
-    // the rest of the node is reversed)
-    if (a.has_next && i == 1) {
-#pragma unroll
-      for (int k = 0; k < 9; ++k)
-        if (CX[k] == 1) {
-          const int sj = j - CY[k];
-          const int sjc = sj < 1 ? 1 : (sj > ny ? ny : sj);
-          const int64_t h = (int64_t)sjc * P;
-          g[2] += (double)(ghost_dT<A>(k, (A)a.uh[h], (A)a.vh[h]) * lam_g(W, k, ox, oy));
-        }
-    }
-    if (!a.core && i <= a.interf) {
-      // initial state: T = t_hot in the liquid strip (initialization.f90:15-26)
-      g[2] += (double)lm.th;
-    }
+    if (edge)
+      adjoint_node<R, EQF, true>(a, W, i, j, ox, oy, g);
+    else
+      adjoint_node<R, EQF, false>(a, W, i, j, ox, oy, g);
   }
   // ---- parameter adjoints: workgroup sum into this workgroup's own slot
   // (butterfly within each wave, then the waves in order: deterministic)
@@ -589,10 +685,10 @@ __global__ __launch_bounds__((BX * tile_rows<R, EQF>()), 5) void k_adjoint(AdjAr
   for (int c = 0; c < 3; ++c)
     for (int m = 32; m > 0; m >>= 1) g[c] += __shfl_xor(g[c], m, 64);
   if ((t & 63) == 0)
-    for (int c = 0; c < 3; ++c) red[c][t >> 6] = g[c];
+    for (int c = 0; c < 3; ++c) red[c][t >> 6] = (double)g[c];
   __syncthreads();
   if (t == 0) {
-    const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    const int64_t b = (int64_t)ty * a.ntx + tx;
     for (int c = 0; c < 3; ++c) {
       double sum = 0.0;
       for (int w = 0; w < NW; ++w) sum += red[c][w];
@@ -908,7 +1004,9 @@ hipError_t launch_adjoint(bgk_handle* h, AdjCtx* x, const StatePtrs& sprev, int 
   a.gm = h->geom;
   a.interf = h->nx / 40;
   const int by = tile_rows_of(h);
-  dim3 grd((unsigned)((h->nx + BX - 1) / BX), (unsigned)((h->ny + by - 1) / by));
+  a.ntx = (h->nx + BX - 1) / BX;
+  a.nty = (h->ny + by - 1) / by;
+  const dim3 grd((unsigned)tile_grid(a.ntx, a.nty));
   const bool f64 = h->prm.precision == BGK_FP64_MOM;
   if (h->mom_eqf && f64)
     hipLaunchKernelGGL((k_adjoint<double, true>), grd, dim3(BX, by), 0, h->stream, a);
