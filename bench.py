@@ -418,8 +418,8 @@ def adjoint_large(nsteps: int = 5000, seg_outer: int = 289, seg: int = 17,
         lat.step(nsteps)
         lat.synchronize()
         t_plain = (time.perf_counter() - t0) / nsteps
-    res = {"workload": f"{w.note}, binomial checkpointing in the free device memory, "
-                       "16 probes",
+    res = {"workload": f"{w.note}, {nsteps} steps, binomial checkpointing in the free "
+                       "device memory, 16 probes",
            "grad": [float(x) for x in g],
            "states_stored": nstates,
            "recomputed_steps": adj_plan(nsteps, nstates - 2),

@@ -91,11 +91,11 @@ def config4() -> Workload:
 
 def config5(n: int = 8192) -> Workload:
     """8192^2 FP32-storage forward + adjoint: FP32 moments, Ra 1e7, Stefan
-    0.1, 5000 steps with two levels of checkpoints; the gradient check is the
-    same workload at 1024^2 against the FP64 adjoint."""
+    0.1, 5000 steps with binomial checkpointing; the gradient check is the
+    same workload at 1024^2 against the FP64 adjoint (two checkpoint levels)."""
     return Workload("config5", cavity(n, 1e7, stefan_number=0.1, rounding=ROUND_CLEAN,
                                       precision=FP32_MOM),
-                    5000, {}, f"{n}^2 adjoint, FP32 moments, Ra 1e7, Stefan 0.1, 5000 steps")
+                    5000, {}, f"{n}^2 adjoint, FP32 moments, Ra 1e7, Stefan 0.1")
 
 
 # the adjoint configurations' parameters (Stefan, k_ratio, T_h in theta units)
