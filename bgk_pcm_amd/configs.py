@@ -52,8 +52,11 @@ def config1() -> Workload:
     conductivity ratio 1, cold right wall and seeded noise, 10000 steps.
     Its Ra = 1e5 needs grav = 0.124 at these relaxation times: a free-fall
     velocity sqrt(grav beta dT H) of 0.9 lattice units (Mach 1.5), and
-    STEP_7 itself reaches NaN before step 1000; grav = 1e-4 is the stable
-    run of the same cavity (Ra 81)."""
+    STEP_7 itself reaches NaN before step 1000.  Measuring buoyancy from a
+    reference temperature (t_ref 273 K or 278 K instead of STEP_7's absolute
+    T) does not cure it: the run is non-finite within 500 steps with FP64
+    populations and with FP32 moments alike (MI355X).  grav = 1e-4 is the
+    stable run of the same cavity (Ra 81)."""
     return Workload("config1", cavity(256, tau_f=0.8, tau_g=15.5, grav=1e-4, stefan_number=0.1,
                                       iter=10000),
                     10000, dict(LITERAL),

@@ -17,13 +17,15 @@ sizes and lengths.
  - config 4, 512^2 adjoint: the gradient of the probe loss against central
    differences of the GPU forward (step 1e-6), within 1e-4.
  - config 5, 8192^2 FP32 forward + adjoint with binomial checkpointing in
-   the memory the device has free fits in HBM and gives a finite gradient;
-   at 1024^2 the FP32 adjoint is within 1e-4 of the FP64 adjoint (two
+   the memory the device has free fits in HBM, gives a finite gradient and
+   costs at most 3x the plain forward steps (recomputations included); at
+   1024^2 the FP32 adjoint is within 1e-4 of the FP64 adjoint (two
    checkpoint levels there).
 """
 import hashlib
 import json
 import pathlib
+import time
 
 import numpy as np
 import pytest
@@ -256,9 +258,23 @@ def test_config5_fp32_adjoint_fits_and_matches_fp64_at_1024():
     print(f"config 5: {nstates} states, {(adj_bytes + info['device_bytes']) / 1e9:.1f} GB of "
           f"{info['device_total'] / 1e9:.1f}; backward", flush=True)
     assert adj_bytes + info["device_bytes"] <= info["device_total"]
+    t0 = time.perf_counter()
     g = lat.adj_backward(2.0 * (rec - target) / rec.size)
+    t_bwd = time.perf_counter() - t0
     lat.close()
     assert np.isfinite(g).all() and np.all(g != 0.0), g
+    # cost of the backward pass (recomputations included) against plain
+    # forward steps of the same lattice
+    with Lattice(w.params) as lat:
+        lat.synchronize()
+        t0 = time.perf_counter()
+        lat.step(w.nsteps)
+        lat.synchronize()
+        t_fwd = time.perf_counter() - t0
+    ratio = t_bwd / t_fwd
+    print(f"config 5: backward {t_bwd / w.nsteps * 1e3:.3f} ms/step, forward "
+          f"{t_fwd / w.nsteps * 1e3:.3f} ms/step: {ratio:.3f}x", flush=True)
+    assert ratio <= 3.0, ratio
     grads = {}
     for prec in (FP32_MOM, FP64_MOM):
         w2 = configs.config5(1024)
