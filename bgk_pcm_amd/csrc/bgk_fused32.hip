@@ -35,10 +35,15 @@
 // The FP32 rounding of ft/gt done by STEP_7 (implicit REAL) is not modelled.
 //
 // Tiling: one node per thread in the XCD-aware row tiles of bgk_device.h;
-// the 9-point phi_prec stencil comes through the L2.  (A row-strip kernel
-// with phi in an LDS ring, as the moment kernel has, ran 30% slower here:
-// 0.61 against 0.47 ms per step at 4096^2, its barrier per row exposing the
-// latency of the population loads.)  Same folding of BC_f/BC_g and the same
+// the 9-point phi_prec stencil comes through the L2.  Each phi row is then
+// fetched into three XCDs' L2s (rows are interleaved over the XCDs), which
+// the HBM counters show as 1.23x the algorithmic reads; staging phi in LDS
+// removes that but costs more than it saves, measured at 4096^2 per step:
+// these row tiles 0.472 ms; a row-strip kernel with phi in an LDS ring, as
+// the moment kernel has, 0.61 ms (its barrier per row exposes the latency
+// of the population loads); 64 x 4 and 64 x 8 tiles with a phi window in
+// LDS, walked in XCD bands, 0.605 and 0.623 ms (a plane's rows are then
+// streamed in 512-byte pieces instead of 2 KB).  Same folding of BC_f/BC_g and the same
 // halo bookkeeping as the FP64 kernel (bgk_fused.hip).
 #include "bgk_internal.h"
 
