@@ -146,36 +146,41 @@ def call_ms(lat, stream, n: int) -> float:
 
 def per_launch(lat, stream, m: int = 200, reps: int = 3):
     """(kernel ms per launch, extra ms of a call's last launch): a call of n
-    steps costs the prologue, n - 1 plain launches and a last launch that
-    also writes every macroscopic field, so T(2m) - T(m) = m plain launches;
-    medians over reps."""
-    k, extra = [], []
+    steps costs a prologue, n - 1 plain launches and a last launch that also
+    writes every macroscopic field, so T(2m) - T(m) = m plain launches
+    (median over reps); the extra is the median time of one-step calls less
+    one plain launch."""
+    k = []
     for _ in range(reps):
         t1 = call_ms(lat, stream, m)
         t2 = call_ms(lat, stream, 2 * m)
         k.append((t2 - t1) / m)
-        extra.append(t1 - m * (t2 - t1) / m)
-    return statistics.median(k), statistics.median(extra)
+    kms = statistics.median(k)
+    one = statistics.median(call_ms(lat, stream, 1) for _ in range(5))
+    return kms, one - kms
 
 
 def measure(p, stream, steps, warmup, kind, kernel, ext=None):
     """MLUPS over `steps` timed steps (wall clock, one bgk_step call after
     `warmup` steps), and the roofline figures of the fused kernel from HIP
-    events on its stream."""
+    events on its stream.  The per-launch timing runs first: its thousands
+    of steps bring the GPU to its working clocks (measured: a 20-step call
+    right after 5 warm-up steps on a GPU that idled takes 1.19x what the
+    next calls take)."""
     from bgk_pcm_amd.lattice import Lattice
     lups = p.nx * p.ny
     with Lattice(p, **(ext or {})) as lat:
         lat.set_stream(stream.cuda_stream)
+        # FP32 populations re-base their reference density every 500 steps:
+        # calls of 500 and 1000 steps hold one and two re-basings, so the
+        # difference is 500 launches plus one re-basing (its amortized cost)
+        kms, extra = per_launch(lat, stream, m=500 if kind == "fp32" else 200)
         lat.step(warmup)
         lat.synchronize()
         t0 = time.perf_counter()
         lat.step(steps)
         lat.synchronize()
         wall = (time.perf_counter() - t0) * 1e3 / steps
-        # FP32 populations re-base their reference density every 500 steps:
-        # calls of 500 and 1000 steps hold one and two re-basings, so the
-        # difference is 500 launches plus one re-basing (its amortized cost)
-        kms, extra = per_launch(lat, stream, m=500 if kind == "fp32" else 200)
         lat.field_check()  # raises on a non-finite field
     nbytes = BYTES_PER_LUP[kind] * lups
     ach = nbytes / (kms * 1e-3) / 1e9

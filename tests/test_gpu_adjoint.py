@@ -152,14 +152,15 @@ def test_adjoint_matches_central_differences():
         assert abs(fd - g[c]) <= 1e-4 * abs(g[c]), (c, fd, g[c])
 
 
-def test_autograd_function_gives_the_adjoint_gradient():
+@pytest.mark.parametrize("n_states", [-1, 5], ids=["segments", "binomial"])
+def test_autograd_function_gives_the_adjoint_gradient(n_states):
     p = Params(nx=24, ny=20, length=24.0, grav=2e-3, rounding=ROUND_CLEAN)
     pi, pj = seeded_probes(p.nx, p.ny, 4, seed=9)
     nsteps = 60
     target = probe_record(p, torch.tensor([0.12, 1.2, 283.2], dtype=torch.float64), pi, pj,
                           nsteps)
     theta = torch.tensor([0.1, 1.5, 283.5], dtype=torch.float64, requires_grad=True)
-    T = probe_temperature(theta, p, pi, pj, nsteps, seg_len=16)
+    T = probe_temperature(theta, p, pi, pj, nsteps, seg_len=16, n_states=n_states)
     loss = ((T - target) ** 2).mean()
     loss.backward()
     th = torch.tensor([0.1, 1.5, 283.5], dtype=torch.float64, requires_grad=True)
