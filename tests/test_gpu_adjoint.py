@@ -327,3 +327,34 @@ def test_binomial_schedule_gives_the_level_schedules_gradient(n_states):
             rec = lat.adj_forward()
             grads.append(lat.adj_backward(2.0 * (rec - target) / rec.size))
     assert np.array_equal(grads[0], grads[1]), grads
+
+
+def test_binomial_schedule_at_1024_gives_the_level_gradient():
+    """At 1024^2 with FP32 moments, with few snapshot slots (leaf loops
+    through the work states as well as split advances) and with every state
+    stored, the binomial schedule's gradient equals the one-level schedule's
+    bit for bit."""
+    from bgk_pcm_amd import configs
+    base = configs.config5(1024).params.replace(precision=FP32_MOM)
+    nsteps = 240
+    pi, pj = seeded_probes(base.nx, base.ny, 16, seed=0)
+    theta0 = configs.ADJ_THETA0
+    target = None
+    grads = []
+    for mode in ("levels", "binomial", "binomial_auto"):
+        print(f"1024^2 adjoint {mode}, {nsteps} steps", flush=True)
+        with Lattice(base.replace(stefan_number=theta0[0]), initialize=False,
+                     t_hot=configs.t_hot_of(base, theta0[2]), k_ratio=theta0[1]) as lat:
+            lat.initialization()
+            if mode == "levels":
+                lat.adj_setup(nsteps, 16, pi, pj)
+            else:
+                lat.adj_setup_binomial(nsteps, 6 if mode == "binomial" else 0, pi, pj)
+            rec = lat.adj_forward()
+            if target is None:
+                target = rec * (1.0 + 1e-4)
+            grads.append(lat.adj_backward(2.0 * (rec - target) / rec.size))
+    assert np.isfinite(grads[0]).all() and np.all(grads[0] != 0.0), grads[0]
+    assert np.array_equal(grads[0], grads[1]), grads
+    assert np.array_equal(grads[0], grads[2]), grads
+
