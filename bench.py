@@ -10,8 +10,9 @@ thermal non-equilibrium moments in FP32; FP64 arithmetic).  One "step" is
 one time step of the whole lattice (BC_f ... moments of the reference time
 loop).  Inputs are resident in HBM before the timed region.
 
-Order of the run: the error runs first (20000 steps of each storage, which
-also bring the GPU to its steady clocks), then the timed steps.
+Order of the run: the error runs first (20000 steps of each storage), then
+for each storage its per-launch timing (a few thousand steps, which bring
+the GPU to its working clocks) followed by its W warm-up and K timed steps.
 
 Printed JSON line (one):
   value          MLUPS of the moment-storage step = Nx*Ny*K/t/1e6 over the K
@@ -21,8 +22,9 @@ Printed JSON line (one):
                  over the fused kernel's time per launch, measured with HIP
                  events on the stream the kernel runs on as the difference of
                  a 400- and a 200-step call (which cancels the last launch of
-                 a call, that also writes every macroscopic field, reported
-                 as last_launch_extra_ms), median of three; vs 8 TB/s
+                 a call, that also writes every macroscopic field), median of
+                 three; vs 8 TB/s.  last_launch_extra_ms: a one-step call
+                 less a plain launch (median of five)
   fp32_population the fused step with FP32 population storage (f -
                  w*rho_ref(row), g - w*T_crit; 160 B per update), same
                  workload and measurements
@@ -47,11 +49,9 @@ Printed JSON line (one):
                  checkpointing with as many snapshots as the free device
                  memory holds, 16 seeded probes): device bytes of the handle
                  and of the adjoint's checkpoints, the forward steps the
-                 backward pass recomputes,
-                 forward and backward time per step and their ratio, the
-                 adjoint kernel's time per launch against the forward
-                 kernel's; and the same gradient by the FP32 and the FP64
-                 adjoint at 1024^2 with their relative difference
+                 backward pass recomputes, forward and backward time per
+                 step and their ratio; and the same gradient by the FP32 and
+                 the FP64 adjoint at 1024^2 with their relative difference
   cpu_baseline   the reference's own STEP_OPENMP time step (compiled from its
                  sources into oracle/_ref) timed on this host's cores on a
                  bounded sample of the same workload; the CPU oracle (a port)
