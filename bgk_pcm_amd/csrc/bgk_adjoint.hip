@@ -81,6 +81,7 @@ struct AdjArgs {
   const double* seeds;  // dL/dT at the probes after step n (null: none)
   double* partial;      // 3 per workgroup: dL/dSt, dL/dk_ratio, dL/dt_hot (accumulated)
   Consts c;
+  ConstsF cf;  // FP32 copies for the FP32 kernel
   Geom gm;
   int interf;
   int ntx, nty;  // tiles across and down
@@ -186,8 +187,8 @@ struct Mom {
 // The moments a forward step stores for a node (bgk_fused_mom.hip's
 // update_node), widened back to FP64: rho and T - T_crit as computed, u, v
 // and the scaled non-equilibrium moments rounded to the storage type R.
-template <typename R, bool EQF, typename A>
-__device__ __forceinline__ Mom<A> stored_mom(const Consts& c, const CoreT<A>& o) {
+template <typename R, bool EQF, typename A, typename C>
+__device__ __forceinline__ Mom<A> stored_mom(const C& c, const CoreT<A>& o) {
   Mom<A> m;
   m.rho = o.rho;
   m.th = o.dT;
@@ -347,8 +348,8 @@ __device__ __forceinline__ void pull_step(const AdjArgs& a, const Win<R, EQF>& W
 // averaged T of moments.f90 seeds half of its adjoint here).  Returns the
 // adjoints of the pulled sums in lS, of the tap sums in lkp, lkm and of the
 // centre tap (direct terms) in lpc; adds parameter adjoints to g[3].
-template <typename A>
-__device__ __forceinline__ void core_backward(const Consts& c, const PullSumsT<A>& S,
+template <typename A, typename C>
+__device__ __forceinline__ void core_backward(const C& c, const PullSumsT<A>& S,
                                               const A tap[9], const CoreT<A>& o, const Mom<A>& lo,
                                               A lphi_o, A ldT2_in, PullSumsT<A>* lS, A* lkp,
                                               A* lkm, A* lpc, A g[3]) {
@@ -380,7 +381,7 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSumsT<A
   // u = u2 + fx s, v = v2 + fy s, s = irho / omega_f
   const A s = o.irho * (A)c.inv_omega_f;
   const A fx = -(o.u2 * (A(1) - o.phi) * A(0.5));
-  const A gb = (A)(c.grav * c.beta);
+  const A gb = (A)c.grav * (A)c.beta;
   const A fy0 = o.rho * gb * (o.T - (A)c.t_ref) * (A(1) + o.phi * A(0.5));
   const A fy = fy0 - o.v2 * (A(1) - o.phi) * A(0.5);
   A lu2 = lu, lv2 = lv;
@@ -401,7 +402,7 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSumsT<A
   lphi += -(A)c.dt_st * A(0.5) * ldT;
   A lpcv = (A)c.dt_st * A(0.5) * ldT;
   // dt_st = delta_T / St
-  g[0] += (-(o.phi - o.pc) * A(0.5) * ldT) * (A)(-c.dt_st * c.inv_stefan);
+  g[0] += (-(o.phi - o.pc) * A(0.5) * ldT) * (-(A)c.dt_st * (A)c.inv_stefan);
   // phi = pc + 0.5 kpr (1 - pc) - 0.5 kmr (1 + pc), kpr = kp/8 rp, kmr = km/8 rm
   const A rp = A(1) - o.rm;
   const A kpr = o.kp_sum * A(0.125) * rp, kmr = o.km_sum * A(0.125) * o.rm;
@@ -428,6 +429,18 @@ __device__ __forceinline__ void core_backward(const Consts& c, const PullSumsT<A
   lS->gxy = lqgxy;
 }
 
+// The constants in the arithmetic type: Consts for FP64, ConstsF for FP32.
+template <typename A>
+struct ConstsOf;
+template <>
+struct ConstsOf<double> {
+  static __device__ __forceinline__ const Consts& of(const AdjArgs& a) { return a.c; }
+};
+template <>
+struct ConstsOf<float> {
+  static __device__ __forceinline__ const ConstsF& of(const AdjArgs& a) { return a.cf; }
+};
+
 // The adjoint of step n at node (i, j), window (ox, oy); the parameter
 // adjoints are added to g.  EDGE = false for the tiles whose nodes and their
 // neighbours are all interior nodes: no boundary terms and no tests.
@@ -451,8 +464,8 @@ __device__ __forceinline__ void adjoint_node(const AdjArgs& a, const Win<R, EQF>
   CoreT<A> o;
   if (a.core) {
     pull_step<R, EQF, A, EDGE>(a, W, i, j, ox, oy, &S, tap);
-    o = core_forward_t<A>(a.c, S, tap);
-    m = stored_mom<R, EQF, A>(a.c, o);
+    o = core_forward_t<A>(ConstsOf<A>::of(a), S, tap);
+    m = stored_mom<R, EQF, A>(ConstsOf<A>::of(a), o);
     ph = o.phi;
   } else {
     m = win_mom<R, EQF, A, typename WinT<R>::A2>(W.a[oy][ox], W.b[oy][ox], W.c[oy][ox]);
@@ -561,7 +574,8 @@ __device__ __forceinline__ void adjoint_node(const AdjArgs& a, const Win<R, EQF>
     // ---- reverse of step n at this node
     PullSumsT<A> lS;
     A lkp, lkm, lpc;
-    core_backward<A>(a.c, S, tap, o, lm, lphi, A(0.5) * lTrep, &lS, &lkp, &lkm, &lpc, g);
+    core_backward<A>(ConstsOf<A>::of(a), S, tap, o, lm, lphi, A(0.5) * lTrep, &lS, &lkp, &lkm,
+                     &lpc, g);
     const AdjPlanes& v = a.cur;
     plane_set<R>(v, L_RHO, q, lS.rho);
     plane_set<R>(v, L_JX, q, lS.jx);
@@ -1001,6 +1015,7 @@ hipError_t launch_adjoint(bgk_handle* h, AdjCtx* x, const StatePtrs& sprev, int 
   a.seeds = seeds;
   a.partial = x->partial;
   a.c = h->c;
+  a.cf = consts_f(h->c);
   a.gm = h->geom;
   a.interf = h->nx / 40;
   const int by = tile_rows_of(h);

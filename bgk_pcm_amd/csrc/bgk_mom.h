@@ -272,16 +272,31 @@ __device__ __forceinline__ float rcp_nr(float d) {
   return fmaf(r, fmaf(-d, r, 1.0f), r);
 }
 
-template <typename A>
-__device__ __forceinline__ A omega_g_of(const Consts& c, A phi) {
+// FP32 copies of the constants the node update and its adjoint read, for
+// the FP32-arithmetic adjoint (filled from Consts on the host, so that no
+// kernel converts them).
+struct ConstsF {
+  float crit_shift, two_over_tw, dt_st, grav, beta, t_ref, inv_omega_f, omega_f, omega_g,
+      alpha_l, k_ratio, inv_stefan;
+};
+inline ConstsF consts_f(const Consts& c) {
+  return ConstsF{(float)c.crit_shift, (float)c.two_over_tw, (float)c.dt_st,   (float)c.grav,
+                 (float)c.beta,       (float)c.t_ref,       (float)c.inv_omega_f,
+                 (float)c.omega_f,    (float)c.omega_g,     (float)c.alpha_l,
+                 (float)c.k_ratio,    (float)c.inv_stefan};
+}
+
+// C: Consts, or ConstsF (the same fields in FP32) for FP32 arithmetic.
+template <typename A, typename C>
+__device__ __forceinline__ A omega_g_of(const C& c, A phi) {
   if (c.k_ratio == 1.0) return (A)c.omega_g;
   const A fl = A(0.5) * (A(1) + phi);
   const A alpha = (A)c.alpha_l * (fl + (A)c.k_ratio * (A(1) - fl));
   return rcp_nr(A(0.5) + alpha * A(3));  // 1/(1/2 + alpha/cs2)
 }
 
-template <typename A>
-__device__ __forceinline__ CoreT<A> core_forward_t(const Consts& c, const PullSumsT<A>& S,
+template <typename A, typename C>
+__device__ __forceinline__ CoreT<A> core_forward_t(const C& c, const PullSumsT<A>& S,
                                                    const A tap[9]) {
   CoreT<A> o;
   o.rho = S.rho;
@@ -323,7 +338,7 @@ __device__ __forceinline__ CoreT<A> core_forward_t(const Consts& c, const PullSu
   o.u = o.u2 + fx * s;
   o.v = o.v2 + fy * s;
 
-  o.omega_g = omega_g_of<A>(c, o.phi);
+  o.omega_g = omega_g_of<A, C>(c, o.phi);
   o.qfxx = S.fxx - A(kCs2) * o.rho - o.rho * o.u * o.u;
   o.qfyy = S.fyy - A(kCs2) * o.rho - o.rho * o.v * o.v;
   o.qfxy = S.fxy - o.rho * o.u * o.v;
@@ -335,7 +350,7 @@ __device__ __forceinline__ CoreT<A> core_forward_t(const Consts& c, const PullSu
 
 __device__ __forceinline__ Core core_forward(const Consts& c, const PullSums& S,
                                              const double tap[9]) {
-  return core_forward_t<double>(c, S, tap);
+  return core_forward_t<double, Consts>(c, S, tap);
 }
 
 // Averaged moments of STEP_7/moments.f90:11-25 from the stored state: sum_k
