@@ -378,7 +378,8 @@ int bgk_create(const bgk_params* p, bgk_handle** out) {
   std::memset(&h->ext, 0, sizeof h->ext);
   h->ext.k_ratio = 1.0;
   // tiling of the moment kernel: BGK_MOM_ROWS (rows per strip; default
-  // about 64, or 32 for tau_f != 1, fitted to the device) and BGK_MOM_VARIANT (1..8: workgroup width 64/128/256/512 with a 4- or
+  // about 64, or 32 for tau_f != 1, fitted to the device) and
+  // BGK_MOM_VARIANT (1..8: workgroup width 64/128/256/512 with a 4- or
   // 3-row ring; 0: chosen from Nx) override the defaults; tuning knobs,
   // every value is correct
   h->strip_rows = 0;
