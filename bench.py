@@ -189,7 +189,7 @@ def measure(p, stream, steps, warmup, kind, kernel, ext=None):
             "last_launch_extra_ms": round(extra, 5),
             "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
             "bytes_per_lup": BYTES_PER_LUP[kind], "algorithmic_bytes_per_launch": nbytes,
-            "traffic": traffic_from_profile(p.nx, kind)}
+            "traffic": traffic_from_profile(p.nx, p.ny, kind)}
 
 
 def rel_errors(a, b):
@@ -282,10 +282,10 @@ def cpu_baseline(p, steps: int):
             "kind": "port", "sample": sample + f"; C oracle with OpenMP, {dt:.2f} s"}
 
 
-def traffic_from_profile(size: int, kind: str):
+def traffic_from_profile(nx: int, ny: int, kind: str):
     """HBM bytes per launch measured with rocprofv3 counters, when a
-    summary for this kernel and size is committed under profiles/."""
-    f = ROOT / "profiles" / f"fused_{kind}_{size}x{size}_hbm_traffic.json"
+    summary for this kernel and lattice is committed under profiles/."""
+    f = ROOT / "profiles" / f"fused_{kind}_{nx}x{ny}_hbm_traffic.json"
     if not f.exists():
         return None
     try:
