@@ -299,9 +299,10 @@ int bgk_adj_info(bgk_handle* h, int64_t* device_bytes, int32_t* n_states);
  * (the initial state's included; 0: as many as the free device memory holds,
  * 2 GiB kept back) plus two working states.  The forward run stores the
  * states the schedule's first branch needs; the backward pass recomputes the
- * minimal number of forward steps for that memory (bgk_adj_plan), about
- * 1.2 per step for 5000 steps with 88 snapshots where two checkpoint levels
- * recompute 2.  bgk_adj_forward and bgk_adj_backward as above; the gradient
+ * minimal number of forward steps for that memory (bgk_adj_plan): 1.05 per
+ * step for 5000 steps with 95 snapshots (8192^2 FP32 moments in an MI355X's
+ * free memory), where two checkpoint levels recompute 2.  bgk_adj_forward
+ * and bgk_adj_backward as above; the gradient
  * equals the level schedules' bit for bit (the same states are recomputed). */
 int bgk_adj_setup_binomial(bgk_handle* h, int32_t nsteps, int32_t n_states, int32_t n_probes,
                            const int32_t* pi, const int32_t* pj);
