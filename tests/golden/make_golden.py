@@ -49,6 +49,10 @@ about two and a half hours on 8 cores):
   cavity256_literal        the 256^2 config (tau_f 0.8, tau_g 15.5, grav
                            1e-4, cold east wall, noise) after 10000 steps:
                            the C oracle
+  cavity4096_ra1e7_pr002_st005  the 4096^2 benchmark cavity (Ra 1e7, Pr 0.02,
+                           Stefan 0.05) as STEP_7 can express it (no
+                           conductivity ratio) after 2000 steps: written
+                           by the reference STEP_OPENMP build
 Each entry also keeps min, max and mean of every field, for a readable
 report when a digest differs.
 
@@ -192,10 +196,14 @@ def long_cases():
     p2 = p2.replace(grav=grav_for_rayleigh(p2, 1e6))
     p1 = Params(nx=256, ny=256, length=256.0, tau_f=0.8, tau_g=15.5, grav=1e-4,
                 stefan_number=0.1, iter=10000)
+    p3 = Params(nx=4096, ny=4096, length=4096.0, tau_f=0.51, tau_g=1.0, stefan_number=0.05,
+                iter=2000)
+    p3 = p3.replace(grav=grav_for_rayleigh(p3, 1e7))
     literal = dict(t_cold=273.0, noise_amp=1e-3, noise_seed=2024)
     return [("cavity256_literal", p1, 10000, literal),
             ("cavity2048_ra1e6_pr002", p2, 20000, None),
-            ("cavity2048_literal", p2, 20000, literal)]
+            ("cavity2048_literal", p2, 20000, literal),
+            ("cavity4096_ra1e7_pr002_st005", p3, 2000, None)]
 
 
 def long_fixture(only=None) -> None:

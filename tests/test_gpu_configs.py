@@ -11,7 +11,10 @@ sizes and lengths.
    STEP_7's arithmetic), bit for bit after the full run, which is how the
    reference validates a run (UTIL/RunMe_thread.sh:7); the literal
    configuration against the oracle's digest.
- - config 3, 4096^2 x 20000 steps: both FP32 storages against the clean FP64
+ - config 3, 4096^2: the FP64 fused step of the cavity as STEP_7 can express
+   it (no conductivity ratio) against the digest of the reference's
+   STEP_OPENMP run after 2000 steps, bit for bit; over 20000 steps (with
+   the conductivity ratio 2) both FP32 storages against the clean FP64
    run from the same initial state, every field within 1e-5 (relative
    max-norm, liquid fraction for phi) after 5000, 10000 and 20000 steps.
  - config 4, 512^2 adjoint: the gradient of the probe loss against central
@@ -35,7 +38,7 @@ import pytest
 from bgk_pcm_amd import configs
 from bgk_pcm_amd.adjoint import seeded_probes
 from bgk_pcm_amd.lattice import Lattice
-from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, FP64, FP64_MOM, ROUND_CLEAN
+from bgk_pcm_amd.params import FP32_DEV, FP32_MOM, FP64, FP64_MOM, ROUND_CLEAN, ROUND_STEP7
 from golden_io import compare_state
 from oracle_py import Oracle
 
@@ -159,6 +162,23 @@ def test_config2_first_steps_against_the_oracle():
 
 
 # ------------------------------------------------------------------ config 3
+
+def test_config3_cavity_fp64_equals_the_reference_after_2000_steps():
+    """Config 3's cavity as STEP_7 can express it (4096^2, Ra 1e7, Pr 0.02,
+    Stefan 0.05; no conductivity ratio): the FP64 fused step against the
+    digest of the reference's own STEP_OPENMP run after 2000 steps, bit for
+    bit, at the benchmark's size."""
+    w = configs.config3()
+    entry = LONG["cavity4096_ra1e7_pr002_st005"]
+    p = w.params.replace(precision=FP64, rounding=ROUND_STEP7)
+    check_params(p, entry)
+    assert entry["extensions"] is None
+    print("config 3 cavity, 4096^2, STEP_7 physics: FP64 fused, "
+          f"{entry['nsteps']} steps vs the reference's digest", flush=True)
+    with Lattice(p) as lat:
+        lat.step(entry["nsteps"])
+        assert digest_mismatches(lat, entry) == []
+
 
 MARKS = (5000, 10000, 20000)
 
