@@ -53,6 +53,10 @@ about two and a half hours on 8 cores):
                            Stefan 0.05) as STEP_7 can express it (no
                            conductivity ratio) after 2000 steps: written
                            by the reference STEP_OPENMP build
+  cavity8192_ra1e7_st01    the 8192^2 cavity of the large adjoint
+                           configuration (Ra 1e7, Stefan 0.1, the reference's
+                           relaxation times) after 500 steps: written by the
+                           reference STEP_OPENMP build
 Each entry also keeps min, max and mean of every field, for a readable
 report when a digest differs.
 
@@ -199,11 +203,14 @@ def long_cases():
     p3 = Params(nx=4096, ny=4096, length=4096.0, tau_f=0.51, tau_g=1.0, stefan_number=0.05,
                 iter=2000)
     p3 = p3.replace(grav=grav_for_rayleigh(p3, 1e7))
+    p5 = Params(nx=8192, ny=8192, length=8192.0, stefan_number=0.1, iter=500)
+    p5 = p5.replace(grav=grav_for_rayleigh(p5, 1e7))
     literal = dict(t_cold=273.0, noise_amp=1e-3, noise_seed=2024)
     return [("cavity256_literal", p1, 10000, literal),
             ("cavity2048_ra1e6_pr002", p2, 20000, None),
             ("cavity2048_literal", p2, 20000, literal),
-            ("cavity4096_ra1e7_pr002_st005", p3, 2000, None)]
+            ("cavity4096_ra1e7_pr002_st005", p3, 2000, None),
+            ("cavity8192_ra1e7_st01", p5, 500, None)]
 
 
 def long_fixture(only=None) -> None:

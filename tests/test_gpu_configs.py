@@ -19,7 +19,9 @@ sizes and lengths.
    max-norm, liquid fraction for phi) after 5000, 10000 and 20000 steps.
  - config 4, 512^2 adjoint: the gradient of the probe loss against central
    differences of the GPU forward (step 1e-6), within 1e-4.
- - config 5, 8192^2: the FP32 moment forward against the FP64 moment
+ - config 5, 8192^2: the FP64 fused step against the digest of the
+   reference's STEP_OPENMP run after 500 steps, bit for bit; the FP32
+   moment forward against the FP64 moment
    forward after its 5000 steps (rho, T, phi within 1e-5, u and v within
    2e-5); FP32 forward + adjoint with binomial checkpointing in
    the memory the device has free fits in HBM, gives a finite gradient and
@@ -267,6 +269,22 @@ def test_config4_adjoint_matches_central_differences():
 
 
 # ------------------------------------------------------------------ config 5
+
+def test_config5_cavity_fp64_equals_the_reference_after_500_steps():
+    """Config 5's cavity at its full size (8192^2, Ra 1e7, Stefan 0.1, the
+    reference's relaxation times): the FP64 fused step against the digest of
+    the reference's own STEP_OPENMP run after 500 steps, bit for bit."""
+    w = configs.config5()
+    entry = LONG["cavity8192_ra1e7_st01"]
+    p = w.params.replace(precision=FP64, rounding=ROUND_STEP7)
+    check_params(p, entry)
+    assert entry["extensions"] is None
+    print(f"config 5 cavity, 8192^2: FP64 fused, {entry['nsteps']} steps vs the "
+          "reference's digest", flush=True)
+    with Lattice(p) as lat:
+        lat.step(entry["nsteps"])
+        assert digest_mismatches(lat, entry) == []
+
 
 def test_config5_fp32_forward_against_fp64_at_8192():
     """Config 5's forward model at its full size and length: the FP32 moment
