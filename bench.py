@@ -26,8 +26,11 @@ Printed JSON line (one):
                  three; vs 8 TB/s.  last_launch_extra_ms: a one-step call
                  less a plain launch (median of five)
   fp32_population the fused step with FP32 population storage (f -
-                 w*rho_ref(row), g - w*T_crit; 160 B per update), same
-                 workload and measurements
+                 w*rho_ref(row, phi), g - w*T_crit; 160 B per update), same
+                 workload and measurements (the kernel's time per launch on
+                 a handle that does not re-base its reference density;
+                 ms_per_launch_with_rebase with the re-basing every 500
+                 steps amortized)
   fp64           the FP64 fused kernel (304 B per update), likewise
   fp32_error     relative max-norm error (per field, liquid fraction for
                  phi) after 5000, 10000 and 20000 steps of the FP32 moment
@@ -169,12 +172,32 @@ def measure(p, stream, steps, warmup, kind, kernel, ext=None):
     next calls take)."""
     from bgk_pcm_amd.lattice import Lattice
     lups = p.nx * p.ny
+    more = {}
+    if kind == "fp32":
+        # FP32 populations re-base their reference density every 500 steps
+        # with two other kernels; the fused kernel's own time per launch
+        # comes from a handle that never re-bases (BGK_REBASE_EVERY=0 at
+        # its creation)
+        old = os.environ.get("BGK_REBASE_EVERY")
+        os.environ["BGK_REBASE_EVERY"] = "0"
+        try:
+            with Lattice(p, **(ext or {})) as lat:
+                lat.set_stream(stream.cuda_stream)
+                kms, extra = per_launch(lat, stream)
+        finally:
+            if old is None:
+                del os.environ["BGK_REBASE_EVERY"]
+            else:
+                os.environ["BGK_REBASE_EVERY"] = old
     with Lattice(p, **(ext or {})) as lat:
         lat.set_stream(stream.cuda_stream)
-        # FP32 populations re-base their reference density every 500 steps:
-        # calls of 500 and 1000 steps hold one and two re-basings, so the
-        # difference is 500 launches plus one re-basing (its amortized cost)
-        kms, extra = per_launch(lat, stream, m=500 if kind == "fp32" else 200)
+        if kind == "fp32":
+            # calls of 500 and 1000 steps hold one and two re-basings: the
+            # time per step with the re-basing amortized
+            kr, _ = per_launch(lat, stream, m=500, reps=1)
+            more["ms_per_launch_with_rebase"] = round(kr, 5)
+        else:
+            kms, extra = per_launch(lat, stream)
         lat.step(warmup)
         lat.synchronize()
         t0 = time.perf_counter()
@@ -189,7 +212,7 @@ def measure(p, stream, steps, warmup, kind, kernel, ext=None):
             "last_launch_extra_ms": round(extra, 5),
             "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
             "bytes_per_lup": BYTES_PER_LUP[kind], "algorithmic_bytes_per_launch": nbytes,
-            "traffic": traffic_from_profile(p.nx, p.ny, kind)}
+            "traffic": traffic_from_profile(p.nx, p.ny, kind), **more}
 
 
 def rel_errors(a, b):
