@@ -65,6 +65,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
                        [--no-fp64] [--no-adjoint] [--no-adjoint-large]
                        [--adjoint-large-steps N] [--adjoint-large-only]
                        [--headline-only] [--calibrate-copy MiB]
+                       [--layout-probe NODES]
 """
 from __future__ import annotations
 
@@ -114,7 +115,46 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="after the timed region, run a device-to-device copy of this size "
                          "(reference kernel for HBM counter scaling in profiles)")
+    ap.add_argument("--layout-probe", type=int, default=0, metavar="NODES",
+                    help="after the timed region, stream the FP32 moment step's 56 bytes in "
+                         "and out per node over three planes and over 64-node tiles "
+                         "(libbgk_calib.so) and report both rates")
     return ap.parse_args()
+
+
+def layout_probe(nodes: int, launches: int = 50) -> dict:
+    """Rate of a kernel that only streams the FP32 moment step's bytes (56 in
+    and 56 out per node, non-temporal 16- and 8-byte accesses), with the
+    three state parts as separate planes and as 64-node tiles
+    (bgk_calib_stream56 in libbgk_calib.so): what each layout allows the
+    step, apart from its arithmetic and its row-strip walk."""
+    import ctypes
+    import torch
+    n = nodes // 64 * 64
+    calib = ctypes.CDLL(str(ROOT / "bgk_pcm_amd" / "lib" / "libbgk_calib.so"))
+    calib.bgk_calib_stream56.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_int, ctypes.c_void_p]
+    src = torch.zeros(56 * n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    out = {"nodes": n, "bytes_per_node": 112}
+    for name, tiled in (("planes_tb_s", 0), ("tiles_tb_s", 1)):
+        def launch():
+            rc = calib.bgk_calib_stream56(src.data_ptr(), dst.data_ptr(), n, tiled,
+                                          stream.cuda_stream)
+            assert rc == 0, rc
+        for _ in range(5):
+            launch()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(launches):
+            launch()
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / launches
+        out[name] = round(112 * n / (ms * 1e-3) / 1e12, 3)
+    del src, dst
+    return out
 
 
 def progress(msg: str) -> None:
@@ -549,6 +589,9 @@ def main():
             res["fp64"]["kernel_ms"] / res["fp32_population"]["kernel_ms"], 3)
     if err is not None:
         res["fp32_error"] = err
+    if args.layout_probe > 0:
+        progress("layout probe")
+        res["layout_probe"] = layout_probe(args.layout_probe)
     if args.headline_only:
         print(json.dumps(res), flush=True)
         return
